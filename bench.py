@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark: spherical Biot-Savart pair interactions per second on one MI355X.
+
+Workload (the metric's configuration, 1.31M sources): icosahedral level 8 --
+655,362 vertex and 1,310,720 face targets, 1,310,720 face sources -- with the
+sum of 32 seeded Gaussian vortices (sigma 0.1, amplitudes N(0,1), mean
+removed) over an Omega = 2 pi background, advanced by the renormalised
+Lagrangian RK4 step (dt = 0.005).  A step is one advance_timestep: four
+direct-sum evaluations over all targets (stages 2-4 and the end-of-step
+solve, which is the next step's stage 1).
+
+value  = FP32C pair interactions per second over the K timed steps, whole
+         job, state resident in HBM, steps replayed from a captured hipGraph.
+Also reported: the FP64 path's rate on the same step, the FP32C/FP64
+speed-up, FP32C error against FP64 for one evaluation, the pair kernel's
+roofline position (HIP events around the kernel on its own stream), and the
+CPU baseline: the oracle's reference-structured loop on a bounded sample.
+
+Pairs per evaluation = (nv + nl) * nl - nl (target x active source, self
+excluded).  Algorithmic flops per pair = 15 (d = y - x: 3, |d|^2: 5,
+q/|d|^2: 1, accumulate s*d: 6).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+FLOPS_PER_PAIR = 15
+FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+DT = 0.005
+OMEGA = 2 * np.pi
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_problem(level: int, seed: int = 1):
+    import lpmkokkos_amd as L
+    mesh = L.TreeMesh(level)
+    lv = mesh.fmask == 0
+    fx = np.ascontiguousarray(mesh.fx[lv])
+    farea = np.ascontiguousarray(mesh.farea[lv])
+    ic = L.VorticityInitialCondition().with_vortices(seed, 32, 1.0, 0.1)
+    vz = ic.eval_many(mesh.vx)
+    fz = ic.eval_many(fx)
+    mu = L.area_mean(fz, farea, np.zeros(len(fz), np.uint8))
+    return mesh, fx, farea, vz - mu, fz - mu
+
+
+def cpu_baseline(mesh, fz_tree, budget_s: float):
+    """Reference-structured CPU loop (oracle): per target, a sequential FP64
+    sweep over all tree faces with the mask, literal 1 - x.y.  Timed on a
+    strided sample of face targets sized to ~budget_s seconds."""
+    from oracle import oracle as O
+    O.lib()
+    nt = mesh.nfaces
+    nl = int((mesh.fmask == 0).sum())
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+
+    def run(stride):
+        t0 = time.perf_counter()
+        O.velocity(mesh.fx, mesh.fx, fz_tree, mesh.farea, mesh.fmask, True, stride=stride)
+        return time.perf_counter() - t0, len(range(0, nt, stride))
+
+    dt, n = run(max(1, nt // (4 * threads)))  # calibration
+    per_target = dt / n
+    want = max(threads, int(budget_s / per_target))
+    stride = max(1, nt // want)
+    dt, n = run(stride)
+    iters = n * mesh.nfaces
+    pairs = n * nl
+    return {
+        "value": pairs / dt,
+        "unit": "pair-interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} face targets (every {stride}th of {nt} tree faces) x {mesh.nfaces} tree-face sources "
+                  f"({nl} active), FP64, {dt:.1f} s; reference loop iterations/s = {iters / dt:.4g}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--targets-per-lane", type=int, default=0)
+    ap.add_argument("--splits", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; the multi-GPU sharding is a design "
+              "(target slices + per-stage source all-gather), not run here. Use --gpus 1.", file=sys.stderr)
+        return 2
+
+    import lpmkokkos_amd as L
+
+    t0 = time.perf_counter()
+    mesh, fx, farea, vz, fz = build_problem(args.level)
+    nv, nl = mesh.nverts, len(fx)
+    pairs_eval = (nv + nl) * nl - nl
+    pairs_step = 4 * pairs_eval
+    log(f"[bench] level {args.level}: {nv} vertices, {nl} active faces, {pairs_eval:.4e} pairs/eval "
+        f"(setup {time.perf_counter() - t0:.1f} s)")
+
+    ctx = L.Context(0)
+    ctx.set_state(mesh.vx, vz, fx, fz, farea, np.zeros(nl, np.uint8))
+    if args.targets_per_lane:
+        ctx.set_launch(args.targets_per_lane, args.splits)
+
+    # FP32C: warmup, then K timed steps replayed from the captured graph
+    ctx.solve(L.PREC_FP32C, L.SCHEME_LAGR)
+    ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=args.warmup)
+    ctx.synchronize()
+    log(f"[bench] warmup done ({args.warmup} steps)")
+    t1 = time.perf_counter()
+    ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=args.steps)
+    ctx.synchronize()
+    t_fp32c = time.perf_counter() - t1
+    ms_step = 1e3 * t_fp32c / args.steps
+    rate = pairs_step * args.steps / t_fp32c
+    log(f"[bench] FP32C: {ms_step:.1f} ms/step, {rate:.4e} pairs/s")
+
+    # pair-kernel launch time with HIP events (one step, eager launches)
+    ctx.enable_timing(True)
+    ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1, use_graph=False)
+    nlaunch, kernel_ms = ctx.pair_timing()
+    ctx.enable_timing(False)
+    achieved = FLOPS_PER_PAIR * pairs_eval / (kernel_ms * 1e-3) / 1e12
+    log(f"[bench] pair kernel: {nlaunch} launches, {kernel_ms:.2f} ms each, {achieved:.2f} TFLOP/s")
+
+    # FP64 on the same step
+    ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=1)  # warm (capture + entry velocity)
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=args.fp64_steps)
+    ctx.synchronize()
+    t_fp64 = time.perf_counter() - t2
+    rate64 = pairs_step * args.fp64_steps / t_fp64
+    log(f"[bench] FP64: {1e3 * t_fp64 / args.fp64_steps:.1f} ms/step, {rate64:.4e} pairs/s")
+
+    # FP32C error against FP64 for one evaluation at the current state
+    ctx.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
+    a = ctx.get_state()
+    ctx.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
+    b = ctx.get_state()
+    u64 = np.concatenate([a["vu"], a["fu"]])
+    u32 = np.concatenate([b["vu"], b["fu"]])
+    p64 = np.concatenate([a["vpsi"], a["fpsi"]])
+    p32 = np.concatenate([b["vpsi"], b["fpsi"]])
+    du = np.linalg.norm(u32 - u64, axis=1)
+    umax = np.linalg.norm(u64, axis=1).max()
+    err = {
+        "velocity_max_rel": float(du.max() / umax),
+        "velocity_rms_rel": float(np.sqrt(np.mean(du ** 2)) / np.sqrt(np.mean(np.sum(u64 ** 2, axis=1)))),
+        "psi_max_rel": float(np.abs(p32 - p64).max() / np.abs(p64).max()),
+        "psi_rms_rel": float(np.sqrt(np.mean((p32 - p64) ** 2) / np.mean(p64 ** 2))),
+    }
+    log(f"[bench] FP32C vs FP64: {err}")
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        mesh_z = np.zeros(mesh.nfaces)
+        mesh_z[mesh.fmask == 0] = fz
+        cpu = cpu_baseline(mesh, mesh_z, args.cpu_seconds)
+        log(f"[bench] CPU baseline: {cpu['value']:.4e} pairs/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "Biot-Savart pair-interactions/s, FP32c vs FP64 at 1.31M src; p50/p99 latency",
+        "value": rate,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"icosahedral level {args.level}: {nv} vertex + {nl} face targets, {nl} face sources; "
+                        "32 Gaussian vortices (seed 1, sigma 0.1) + Omega=2pi; Lagrangian RK4 dt=0.005",
+            "pairs_per_step": pairs_step,
+            "evaluations_per_step": 4,
+        },
+        "rk4_steps_per_s": 1e3 / ms_step,
+        "fp64_pairs_per_s": rate64,
+        "fp32c_vs_fp64_speedup": rate / rate64,
+        "fp32c_error_vs_fp64": err,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "k_pair_f32c",
+            "kernel_ms": kernel_ms,
+            "flops_per_pair": FLOPS_PER_PAIR,
+            "pairs_per_launch": pairs_eval,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,179 @@
+/*
+ * lpm_bve.h -- C ABI of the MI355X-native BVE Biot-Savart library
+ * (liblpm_bve.so).
+ *
+ * Plain pointers and sizes only.  Every call returns an lpm_status; on a
+ * non-zero status lpm_last_error() (thread-local) describes the failure.
+ * No exception crosses this boundary.
+ *
+ * Layouts: positions and velocities are [n][3] row-major doubles (the
+ * reference's host layout of View<double*[3]>, src/LpmDefs.hpp:43-47);
+ * scalars are [n] doubles; masks are [n] bytes (non-zero = divided face,
+ * excluded as a source, src/LpmFaces.cpp:244-247).  "Tree faces" means all
+ * faces of the refinement tree, parents included (src/LpmBVERK4.cpp:21-36).
+ */
+#ifndef LPM_BVE_H
+#define LPM_BVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LPM_OK = 0,
+  LPM_ERR_ARG = 1,     /* invalid argument or size */
+  LPM_ERR_HIP = 2,     /* HIP runtime error */
+  LPM_ERR_NOMEM = 3,   /* device allocation failed */
+  LPM_ERR_STATE = 4,   /* call not valid in the current state */
+  LPM_ERR_BUDGET = 5,  /* request exceeds the admission memory budget */
+  LPM_ERR_IO = 6       /* file could not be read */
+} lpm_status;
+
+/* Arithmetic of the pair sums. */
+typedef enum {
+  LPM_PREC_FP64 = 0,  /* FP64 throughout */
+  LPM_PREC_FP32C = 1  /* FP32 pair terms from tile-local offsets, FP64 tile partials */
+} lpm_prec;
+
+/* Time-stepping scheme. */
+typedef enum {
+  /* The reference step as written (src/LpmBVERK4_Impl.hpp:42-130): kernel
+   * denominator 1 - x.y, vorticity integrated with dzeta = -2 Omega w dt, the
+   * face-vorticity combination (1,2,4,4) of :120-121, no renormalisation. */
+  LPM_SCHEME_REF = 0,
+  /* Projected RK4: stage and updated positions normalised to the unit
+   * sphere, zeta = (zeta0 + 2 Omega z0) - 2 Omega z, denominator |y-x|^2/2. */
+  LPM_SCHEME_LAGR = 1
+} lpm_scheme;
+
+const char* lpm_last_error(void);
+int lpm_version(void); /* major*10000 + minor*100 + patch */
+int lpm_device_count(int* count);
+
+/* ------------------------------------------------------------------ */
+/* Mesh build (host).                                                  */
+/* ------------------------------------------------------------------ */
+
+/* MeshSeed<IcosTriSphereSeed>::setMaxAllocations (src/LpmMeshSeed.cpp:296-305):
+ * vertex, edge and tree-face counts of a level-`level` icosahedral tree. */
+int lpm_icos_alloc(int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces);
+
+/* PolyMesh2d<IcosTriSphereSeed>::treeInit (src/LpmPolyMesh2d.cpp:30-44) with
+ * the built-in seed (seed_path == NULL) or a seed file in the reference's
+ * format (src/LpmMeshSeed.cpp:20-232).  Arrays are sized by lpm_icos_alloc:
+ * vx, vlag [nverts][3]; fx, flag [nfaces][3]; farea, fmask [nfaces];
+ * fverts [nfaces][3]; fparent [nfaces].  Any output may be NULL.
+ * counts (non-NULL) receives {nverts, nedges, nfaces, nleaves}. */
+int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vlag, double* fx, double* flag,
+                       double* farea, uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts);
+
+/* ------------------------------------------------------------------ */
+/* Vorticity initial conditions (host), the                           */
+/* VorticityInitialCondition::eval(x,y,z) interface of                 */
+/* src/LpmVorticityGallery.hpp:13-40, as one composite field:          */
+/*   zeta = sbr * 2 * (2 pi) * z                       (SolidBodyRotation)  */
+/*        + 2 w sin(lat) - K (n+1)(n+2) cos^n(lat) sin(lat) cos(n lon) (RH, n>0) */
+/*        + sum_k amp_k exp(-(1 - x.c_k) / sigma^2)    (Gaussian vortices)  */
+/* ------------------------------------------------------------------ */
+
+/* Seeded vortex parameters: SplitMix64 -> xoshiro256**, Box-Muller normals;
+ * for each k: three normals normalised to the sphere (centre), then
+ * amp_scale times one normal (amplitude). centers [nvort][3], amps [nvort]. */
+int lpm_ic_gauss_params(uint64_t seed, int nvort, double amp_scale, double* centers, double* amps);
+
+int lpm_ic_eval(const double* xyz, int64_t n, double* zeta, double sbr, int rh_n, double rh_w, double rh_K,
+                int nvort, const double* centers, const double* amps, double sigma);
+
+/* Area-weighted mean of zeta over unmasked faces, summed in index order. */
+int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, double* mean);
+
+/* ------------------------------------------------------------------ */
+/* Device functors (device pointers, asynchronous on `stream`, NULL =  */
+/* the null stream).  FP64, denominator 1 - x.y, as the reference.     */
+/* Scratch device memory is taken from a per-device workspace that     */
+/* grows on demand; calls on one device must not overlap.              */
+/* ------------------------------------------------------------------ */
+
+/* BVEVertexVelocity (src/LpmBVEKernels.hpp:163-186): u[nv][3] at targets
+ * vx[nv][3] from the unmasked faces among fx[nf][3]. */
+int lpm_bve_vertex_velocity(double* u, const double* vx, int64_t nv, const double* fx, const double* fzeta,
+                            const double* farea, const uint8_t* fmask, int64_t nf, void* stream);
+/* BVEFaceVelocity (src/LpmBVEKernels.hpp:285-305): u[nf][3] at every face,
+ * the face itself excluded as a source. */
+int lpm_bve_face_velocity(double* u, const double* fx, const double* fzeta, const double* farea,
+                          const uint8_t* fmask, int64_t nf, void* stream);
+/* BVEVertexStreamFn / BVEFaceStreamFn (src/LpmBVEKernels.hpp:140-161, 263-283). */
+int lpm_bve_vertex_streamfn(double* psi, const double* vx, int64_t nv, const double* fx, const double* fzeta,
+                            const double* farea, const uint8_t* fmask, int64_t nf, void* stream);
+int lpm_bve_face_streamfn(double* psi, const double* fx, const double* fzeta, const double* farea,
+                          const uint8_t* fmask, int64_t nf, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Solver context: the BVESphere fields and BVERK4 stage buffers of    */
+/* one problem, resident in device memory (src/LpmBVESphere.hpp:18-80, */
+/* src/LpmBVERK4.hpp:13-65).  One host thread uses a context at a      */
+/* time; contexts may run concurrently, each on its own stream.        */
+/* ------------------------------------------------------------------ */
+typedef struct lpm_bve_ctx lpm_bve_ctx;
+
+int lpm_bve_create(int device, lpm_bve_ctx** ctx);
+int lpm_bve_destroy(lpm_bve_ctx* ctx);
+
+/* Upload a particle state (host arrays; copied).  vx [nv][3], vzeta [nv];
+ * fx [nf][3], fzeta, farea, fmask [nf] over tree faces.  Velocities become
+ * invalid until lpm_bve_solve, lpm_bve_set_velocity or the first step. */
+int lpm_bve_set_state(lpm_bve_ctx* ctx, int64_t nv, const double* vx, const double* vzeta, int64_t nf,
+                      const double* fx, const double* fzeta, const double* farea, const uint8_t* fmask);
+
+/* Provide the entry velocity (the BVERK4 precondition, F1: velocity valid at
+ * the current state).  vu [nv][3], fu [nf][3]. */
+int lpm_bve_set_velocity(lpm_bve_ctx* ctx, const double* vu, const double* fu);
+
+/* Velocity (and, if want_psi, stream function) at the current state, as the
+ * solves of BVESphere::init_vorticity (src/LpmBVESphere.cpp:151-168).
+ * scheme selects the kernel denominator (REF: 1 - x.y; LAGR: |y-x|^2/2). */
+int lpm_bve_solve(lpm_bve_ctx* ctx, int prec, int scheme, int want_psi);
+
+/* `nsteps` calls of BVERK4::advance_timestep (src/LpmBVERK4_Impl.hpp:42-130)
+ * on the resident state.  On return the velocity is valid at the new state.
+ * LPM_PREC_FP32C requires LPM_SCHEME_LAGR.  Steps are replayed from a
+ * captured hipGraph when use_graph != 0. */
+int lpm_bve_advance(lpm_bve_ctx* ctx, double dt, double omega, int scheme, int prec, int nsteps, int use_graph);
+
+/* Copy the resident state to host arrays (any pointer may be NULL).
+ * Blocks until the context's stream has drained. */
+int lpm_bve_get_state(lpm_bve_ctx* ctx, double* vx, double* vzeta, double* vu, double* vpsi, double* fx,
+                      double* fzeta, double* fu, double* fpsi);
+
+/* Total circulation sum over unmasked faces of zeta * area (FP64, device). */
+int lpm_bve_circulation(lpm_bve_ctx* ctx, double* circ);
+
+/* Block until all work queued on the context has finished. */
+int lpm_bve_synchronize(lpm_bve_ctx* ctx);
+
+/* Sizes of the resident problem: {nv, nf, nsources, ntargets}. */
+int lpm_bve_sizes(lpm_bve_ctx* ctx, int64_t* sizes4);
+
+/* Kernel launch shape: targets per lane (1, 2 or 4) and source splits
+ * (0 = automatic).  Applies to later solves and steps. */
+int lpm_bve_set_launch(lpm_bve_ctx* ctx, int targets_per_lane, int source_splits);
+
+/* The context's HIP stream (hipStream_t) and device id. */
+int lpm_bve_stream(lpm_bve_ctx* ctx, void** stream, int* device);
+
+/* Device time of the most recent pair-sum kernel launches queued by this
+ * context (HIP events on the context's stream): launches, mean milliseconds. */
+int lpm_bve_last_pair_timing(lpm_bve_ctx* ctx, int* launches, double* mean_ms);
+
+/* Record HIP events around every pair-sum kernel this context queues (1) or
+ * stop (0).  While on, steps run without graph replay. */
+int lpm_bve_enable_timing(lpm_bve_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LPM_BVE_H */

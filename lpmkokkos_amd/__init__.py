@@ -1,0 +1,8 @@
+"""MI355X-native BVE Biot-Savart direct sum (the hot path of pbosler/lpmKokkos).
+
+The compute lives in liblpm_bve.so (hand-written HIP for gfx950 behind the C
+ABI of include/lpm_bve.h); this package is its Python front end.
+"""
+from ._lib import PREC_FP32C, PREC_FP64, SCHEME_LAGR, SCHEME_REF, LpmError, lib  # noqa: F401
+from .bve import (BVERK4, BVESphere, Context, IcosTriSphereSeed, MeshSeed, RossbyHaurwitzWave,  # noqa: F401
+                  SolidBodyRotation, TreeMesh, VorticityInitialCondition, area_mean, gauss_params)

@@ -1,0 +1,101 @@
+"""ctypes binding of liblpm_bve.so (the C ABI declared in include/lpm_bve.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (make in
+lpmkokkos_amd/csrc).  There is no fallback: if the library is missing,
+loading raises, so nothing here ever computes on the CPU in its place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "liblpm_bve.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "lpm_bve.h")
+
+LPM_OK = 0
+PREC_FP64, PREC_FP32C = 0, 1
+SCHEME_REF, SCHEME_LAGR = 0, 1
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_dpn = C.c_void_p  # nullable double*
+_bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+# (name, argtypes) for every entry point of include/lpm_bve.h
+SIGNATURES = {
+    "lpm_last_error": [],
+    "lpm_version": [],
+    "lpm_device_count": [C.POINTER(C.c_int)],
+    "lpm_icos_alloc": [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+    "lpm_icos_tree_init": [C.c_int, C.c_char_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
+    "lpm_ic_gauss_params": [C.c_uint64, C.c_int, C.c_double, _dp, _dp],
+    "lpm_ic_eval": [_dp, C.c_int64, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dpn, _dpn,
+                    C.c_double],
+    "lpm_area_mean": [_dp, _dp, _bp, C.c_int64, C.POINTER(C.c_double)],
+    "lpm_bve_vertex_velocity": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int64, C.c_void_p],
+    "lpm_bve_face_velocity": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+    "lpm_bve_vertex_streamfn": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int64, C.c_void_p],
+    "lpm_bve_face_streamfn": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+    "lpm_bve_create": [C.c_int, C.POINTER(C.c_void_p)],
+    "lpm_bve_destroy": [C.c_void_p],
+    "lpm_bve_set_state": [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _bp],
+    "lpm_bve_set_velocity": [C.c_void_p, _dp, _dp],
+    "lpm_bve_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+    "lpm_bve_advance": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
+    "lpm_bve_get_state": [C.c_void_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
+    "lpm_bve_circulation": [C.c_void_p, C.POINTER(C.c_double)],
+    "lpm_bve_synchronize": [C.c_void_p],
+    "lpm_bve_sizes": [C.c_void_p, _i64],
+    "lpm_bve_set_launch": [C.c_void_p, C.c_int, C.c_int],
+    "lpm_bve_stream": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)],
+    "lpm_bve_last_pair_timing": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "lpm_bve_enable_timing": [C.c_void_p, C.c_int],
+}
+
+_lib = None
+
+
+class LpmError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Load liblpm_bve.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C lpmkokkos_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.lpm_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(fn: str, rc: int) -> None:
+    if rc != LPM_OK:
+        raise LpmError(fn, rc, lib().lpm_last_error().decode(errors="replace"))
+
+
+def call(name: str, *args) -> None:
+    check(name, getattr(lib(), name)(*args))
+
+
+def ptr(a) -> int | None:
+    """Data pointer of a numpy array or torch tensor (None passes NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()

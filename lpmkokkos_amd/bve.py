@@ -1,0 +1,353 @@
+"""Host-side mirror of the reference's BVE interface over the C ABI.
+
+Names, argument meaning and call order follow the reference:
+
+    seed = MeshSeed(IcosTriSphereSeed)                      # src/LpmMeshSeed.hpp:181-224
+    nv, ne, nf = seed.setMaxAllocations(depth)              # src/LpmMeshSeed.cpp:296-305
+    sphere = BVESphere(nv, ne, nf)                          # src/LpmBVESphere.hpp:40
+    sphere.treeInit(depth, seed)                            # src/LpmPolyMesh2d.hpp:171
+    sphere.set_omega(omega)                                 # src/LpmBVESphere.cpp:115-124
+    sphere.init_vorticity(RossbyHaurwitzWave(4, w, K))      # src/LpmBVESphere.cpp:126-169
+    solver = BVERK4(dt, Omega); solver.init(nv, nf)         # src/LpmBVERK4.hpp:29-34
+    solver.advance_timestep(sphere.physVerts.crds, sphere.relVortVerts, sphere.velocityVerts,
+                            sphere.physFaces.crds, sphere.relVortFaces, sphere.velocityFaces,
+                            sphere.faces.area, sphere.faces.mask)
+
+The particle state lives in device memory (one lpm_bve_ctx per sphere);
+the view objects name fields of that state, and ``updateHost`` copies them
+back, as the reference's host mirrors do.  All arithmetic runs in
+liblpm_bve.so (host mesh build, host IC evaluation, device pair sums).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import PREC_FP32C, PREC_FP64, SCHEME_LAGR, SCHEME_REF, call
+
+PI = 3.1415926535897932384626433832795027975
+
+
+# ---------------------------------------------------------------------------
+# mesh
+# ---------------------------------------------------------------------------
+class IcosTriSphereSeed:
+    """src/LpmMeshSeed.hpp:132-147."""
+    nverts, nfaces, nedges, nfaceverts = 12, 20, 30, 3
+
+    @staticmethod
+    def idString() -> str:
+        return "IcosTriSphereSeed"
+
+    @staticmethod
+    def nVerticesAtTreeLevel(lev: int) -> int:
+        return 2 + 10 * 4 ** lev
+
+    @staticmethod
+    def nFacesAtTreeLevel(lev: int) -> int:
+        return 20 * 4 ** lev
+
+
+class MeshSeed:
+    """MeshSeed<SeedType>: the built-in icosahedral seed, or a seed file in the
+    reference's format when ``path`` is given."""
+
+    def __init__(self, seed_type=IcosTriSphereSeed, path: str | None = None):
+        if seed_type is not IcosTriSphereSeed:
+            raise NotImplementedError("only IcosTriSphereSeed is built")
+        self.seed_type = seed_type
+        self.path = path
+
+    def idString(self) -> str:
+        return self.seed_type.idString()
+
+    def setMaxAllocations(self, lev: int):
+        nv, ne, nf = C.c_int64(), C.c_int64(), C.c_int64()
+        call("lpm_icos_alloc", lev, C.byref(nv), C.byref(ne), C.byref(nf))
+        return nv.value, ne.value, nf.value
+
+
+class TreeMesh:
+    """Arrays of PolyMesh2d after treeInit (host copies)."""
+
+    def __init__(self, level: int, seed: MeshSeed | None = None):
+        seed = seed or MeshSeed()
+        nv, ne, nf = seed.setMaxAllocations(level)
+        self.level = level
+        self.vx = np.zeros((nv, 3))
+        self.vlag = np.zeros((nv, 3))
+        self.fx = np.zeros((nf, 3))
+        self.flag = np.zeros((nf, 3))
+        self.farea = np.zeros(nf)
+        self.fmask = np.zeros(nf, np.uint8)
+        self.fverts = np.zeros((nf, 3), np.int32)
+        self.fparent = np.zeros(nf, np.int32)
+        counts = np.zeros(4, np.int64)
+        call("lpm_icos_tree_init", level, seed.path.encode() if seed.path else None,
+             *(_lib.ptr(a) for a in (self.vx, self.vlag, self.fx, self.flag, self.farea, self.fmask, self.fverts,
+                                     self.fparent, counts)))
+        self.nverts, self.nedges, self.nfaces, self.nleaves = (int(c) for c in counts)
+
+    def leaf_index(self):
+        return np.nonzero(self.fmask == 0)[0]
+
+
+# ---------------------------------------------------------------------------
+# initial conditions: VorticityInitialCondition::eval (src/LpmVorticityGallery.hpp:13-40)
+# ---------------------------------------------------------------------------
+class VorticityInitialCondition:
+    """Composite field: solid-body rotation, Rossby-Haurwitz wave, Gaussian vortices
+    (evaluated by lpm_ic_eval in liblpm_bve.so)."""
+
+    def __init__(self, sbr: float = 0.0, rh_n: int = 0, rh_w: float = 0.0, rh_K: float = 0.0,
+                 centers=None, amps=None, sigma: float = 1.0):
+        self.sbr, self.rh_n, self.rh_w, self.rh_K, self.sigma = sbr, rh_n, rh_w, rh_K, sigma
+        self.centers = None if centers is None else np.ascontiguousarray(centers, np.float64).reshape(-1, 3)
+        self.amps = None if amps is None else np.ascontiguousarray(amps, np.float64)
+
+    def with_vortices(self, seed: int, nvort: int, amp_scale: float, sigma: float):
+        c, a = gauss_params(seed, nvort, amp_scale)
+        self.centers, self.amps, self.sigma = c, a, sigma
+        return self
+
+    def eval_many(self, xyz) -> np.ndarray:
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.zeros(xyz.shape[0])
+        nv = 0 if self.amps is None else len(self.amps)
+        call("lpm_ic_eval", xyz, xyz.shape[0], out, float(self.sbr), int(self.rh_n), float(self.rh_w),
+             float(self.rh_K), nv, _lib.ptr(self.centers) if nv else None, _lib.ptr(self.amps) if nv else None,
+             float(self.sigma))
+        return out
+
+    def eval(self, x: float, y: float, z: float) -> float:
+        return float(self.eval_many(np.array([[x, y, z]]))[0])
+
+
+class SolidBodyRotation(VorticityInitialCondition):
+    """zeta = 2 * OMEGA * z, OMEGA = 2 pi (src/LpmVorticityGallery.hpp:25-40)."""
+    OMEGA = 2 * PI
+
+    def __init__(self):
+        super().__init__(sbr=1.0)
+
+
+class RossbyHaurwitzWave(VorticityInitialCondition):
+    """zeta = 2 w sin(lat) - K (n+1)(n+2) cos^n(lat) sin(lat) cos(n lon)."""
+
+    def __init__(self, n: int, w: float, K: float):
+        super().__init__(rh_n=n, rh_w=w, rh_K=K)
+
+    def phase_speed(self, omega: float) -> float:
+        n, w = self.rh_n, self.rh_w
+        return (n * (n + 3) * w - 2 * omega) / ((n + 1) * (n + 2))
+
+
+def gauss_params(seed: int, nvort: int, amp_scale: float = 1.0):
+    c = np.zeros((nvort, 3))
+    a = np.zeros(nvort)
+    if nvort:
+        call("lpm_ic_gauss_params", seed, nvort, amp_scale, c, a)
+    return c, a
+
+
+def area_mean(fzeta, farea, fmask) -> float:
+    m = C.c_double()
+    call("lpm_area_mean", np.ascontiguousarray(fzeta, np.float64), np.ascontiguousarray(farea, np.float64),
+         np.ascontiguousarray(fmask, np.uint8), len(fzeta), C.byref(m))
+    return m.value
+
+
+# ---------------------------------------------------------------------------
+# device state
+# ---------------------------------------------------------------------------
+class Context:
+    """One lpm_bve_ctx: a particle state resident in device memory."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        call("lpm_bve_create", device, C.byref(h))
+        self.h = h
+        self.device = device
+        self.nv = self.nf = 0
+
+    def close(self):
+        if self.h:
+            _lib.lib().lpm_bve_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, vx, vzeta, fx, fzeta, farea, fmask):
+        vx = np.ascontiguousarray(vx, np.float64).reshape(-1, 3)
+        fx = np.ascontiguousarray(fx, np.float64).reshape(-1, 3)
+        self.nv, self.nf = vx.shape[0], fx.shape[0]
+        call("lpm_bve_set_state", self.h, self.nv, vx, np.ascontiguousarray(vzeta, np.float64), self.nf, fx,
+             np.ascontiguousarray(fzeta, np.float64), np.ascontiguousarray(farea, np.float64),
+             np.ascontiguousarray(fmask, np.uint8))
+
+    def set_velocity(self, vu, fu):
+        call("lpm_bve_set_velocity", self.h, np.ascontiguousarray(vu, np.float64),
+             np.ascontiguousarray(fu, np.float64))
+
+    def solve(self, prec: int = PREC_FP64, scheme: int = SCHEME_REF, want_psi: bool = False):
+        call("lpm_bve_solve", self.h, prec, scheme, int(want_psi))
+
+    def advance(self, dt: float, omega: float, scheme: int = SCHEME_REF, prec: int = PREC_FP64, nsteps: int = 1,
+                use_graph: bool = True):
+        call("lpm_bve_advance", self.h, dt, omega, scheme, prec, nsteps, int(use_graph))
+
+    def set_launch(self, targets_per_lane: int, source_splits: int = 0):
+        call("lpm_bve_set_launch", self.h, targets_per_lane, source_splits)
+
+    def enable_timing(self, on: bool):
+        call("lpm_bve_enable_timing", self.h, int(on))
+
+    def pair_timing(self):
+        n, ms = C.c_int(), C.c_double()
+        call("lpm_bve_last_pair_timing", self.h, C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
+    def synchronize(self):
+        call("lpm_bve_synchronize", self.h)
+
+    def sizes(self):
+        s = np.zeros(4, np.int64)
+        call("lpm_bve_sizes", self.h, s)
+        return tuple(int(v) for v in s)
+
+    def circulation(self) -> float:
+        c = C.c_double()
+        call("lpm_bve_circulation", self.h, C.byref(c))
+        return c.value
+
+    def get_state(self):
+        nv, nf = self.nv, self.nf
+        out = dict(vx=np.zeros((nv, 3)), vzeta=np.zeros(nv), vu=np.zeros((nv, 3)), vpsi=np.zeros(nv),
+                   fx=np.zeros((nf, 3)), fzeta=np.zeros(nf), fu=np.zeros((nf, 3)), fpsi=np.zeros(nf))
+        call("lpm_bve_get_state", self.h, *(_lib.ptr(out[k]) for k in
+                                            ("vx", "vzeta", "vu", "vpsi", "fx", "fzeta", "fu", "fpsi")))
+        return out
+
+
+class DeviceView:
+    """Handle naming one field of a sphere's resident state (the role of a
+    Kokkos View handle in BVERK4::advance_timestep's signature)."""
+
+    def __init__(self, sphere: "BVESphere", name: str):
+        self.sphere, self.name = sphere, name
+
+
+class _Coords:
+    def __init__(self, sphere, which):
+        self.crds = DeviceView(sphere, which + "x")
+
+
+class _Faces:
+    def __init__(self, sphere):
+        self.area = DeviceView(sphere, "farea")
+        self.mask = DeviceView(sphere, "fmask")
+
+
+class BVESphere:
+    """BVESphere<IcosTriSphereSeed> (src/LpmBVESphere.hpp:18-80) with fields
+    resident on device `device`."""
+
+    def __init__(self, nmaxverts: int, nmaxedges: int, nmaxfaces: int, nq: int = 0, device: int = 0):
+        self.nmax = (nmaxverts, nmaxedges, nmaxfaces)
+        self.Omega = 2 * PI  # src/LpmBVESphere.cpp:24
+        self._omg_set = False
+        self.t = 0.0
+        self.ctx = Context(device)
+        self.mesh: TreeMesh | None = None
+        self.physVerts = _Coords(self, "v")
+        self.physFaces = _Coords(self, "f")
+        self.faces = _Faces(self)
+        self.relVortVerts = DeviceView(self, "vzeta")
+        self.relVortFaces = DeviceView(self, "fzeta")
+        self.velocityVerts = DeviceView(self, "vu")
+        self.velocityFaces = DeviceView(self, "fu")
+        self.host = {}
+
+    # PolyMesh2d::treeInit
+    def treeInit(self, depth: int, seed: MeshSeed):
+        self.mesh = TreeMesh(depth, seed)
+        if self.mesh.nverts > self.nmax[0] or self.mesh.nfaces > self.nmax[2]:
+            raise ValueError("treeInit: mesh exceeds the allocation sizes")
+
+    def nvertsHost(self) -> int:
+        return self.mesh.nverts
+
+    def nfacesHost(self) -> int:
+        return self.mesh.nfaces
+
+    def set_omega(self, omg: float):
+        if self._omg_set:
+            print(f"BVESphere::set_omega warning: omega = {self.Omega} already set.")
+        else:
+            self.Omega = omg
+            self._omg_set = True
+
+    def init_vorticity(self, ic: VorticityInitialCondition, subtract_mean: bool = False, prec: int = PREC_FP64,
+                       scheme: int = SCHEME_REF):
+        """Evaluate the IC at every particle (host), set absVort = zeta + 2 Omega z,
+        upload, and compute stream function and velocity on the device."""
+        m = self.mesh
+        vz = ic.eval_many(m.vx)
+        fz = ic.eval_many(m.fx)
+        if subtract_mean:
+            mu = area_mean(fz, m.farea, m.fmask)
+            vz = vz - mu
+            fz = fz - mu
+        self.host["absVortVerts"] = vz + 2 * self.Omega * m.vx[:, 2]
+        self.host["absVortFaces"] = fz + 2 * self.Omega * m.fx[:, 2]
+        self.ctx.set_state(m.vx, vz, m.fx, fz, m.farea, m.fmask)
+        self.ctx.solve(prec, scheme, want_psi=True)
+
+    def updateHost(self):
+        s = self.ctx.get_state()
+        self.host.update(s)
+        return s
+
+    def avg_mesh_size_radians(self) -> float:
+        return math.sqrt(4 * PI / self.nfacesHost())
+
+    def avg_mesh_size_degrees(self) -> float:
+        return math.degrees(self.avg_mesh_size_radians())
+
+    def circulation(self) -> float:
+        return self.ctx.circulation()
+
+
+class BVERK4:
+    """BVERK4 (src/LpmBVERK4.hpp:13-65): classical RK4 over the sphere's
+    resident state.  scheme/prec select the reference step (REF, FP64) or the
+    renormalised Lagrangian step in FP64 or FP32C."""
+
+    def __init__(self, timestep: float, omg: float, scheme: int = SCHEME_REF, prec: int = PREC_FP64,
+                 use_graph: bool = True):
+        self.dt, self.Omega = timestep, omg
+        self.scheme, self.prec, self.use_graph = scheme, prec, use_graph
+        self.nverts = self.nfaces = 0
+
+    def init(self, nv: int, nf: int):
+        self.nverts, self.nfaces = nv, nf
+
+    def advance_timestep(self, vx, vzeta, vvel, fx, fzeta, fvel, fa, fm, nsteps: int = 1):
+        views = (vx, vzeta, vvel, fx, fzeta, fvel, fa, fm)
+        names = ("vx", "vzeta", "vu", "fx", "fzeta", "fu", "farea", "fmask")
+        sphere = vx.sphere
+        for v, n in zip(views, names):
+            if not isinstance(v, DeviceView) or v.sphere is not sphere or v.name != n:
+                raise ValueError("advance_timestep: arguments must be the sphere's "
+                                 "(vx, vzeta, vvel, fx, fzeta, fvel, area, mask) views, in that order")
+        if sphere.mesh.nverts != self.nverts or sphere.mesh.nfaces != self.nfaces:
+            raise ValueError("advance_timestep: init(nverts, nfaces) does not match the sphere")
+        sphere.ctx.advance(self.dt, self.Omega, self.scheme, self.prec, nsteps, self.use_graph)
+        sphere.t += nsteps * self.dt

@@ -1,0 +1,1237 @@
+// MI355X (gfx950) implementation of the BVE Biot-Savart hot path and the
+// C ABI of include/lpm_bve.h.  Kernel design notes: lpm_bve_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "lpm_bve.h"
+#include "lpm_bve_kernels.hpp"
+#include "lpm_status.hpp"
+
+namespace lpm {
+namespace dev {
+
+constexpr double kPi = 3.1415926535897932384626433832795027975;  // src/LpmDefs.hpp:82
+constexpr double kLn2 = 0.69314718055994530941723212145817657;
+
+// ---------------------------------------------------------------------------
+// scalar helpers
+// ---------------------------------------------------------------------------
+
+// Reciprocal: hardware estimate refined by two Newton steps (error ~1 ulp).
+__device__ __forceinline__ double rcp64(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// source preparation
+// ---------------------------------------------------------------------------
+
+// FP64 source records {x, y, z, q}.  Source j is target smap[j]; its area is
+// area[smap[j]].  FORM_DOT: q = (-zeta A)/(4 pi); FORM_CHORD: q = (-zeta A)/(2 pi).
+// Padding records have q = 0 at the origin (denominator 1 in both forms).
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_prep_f64(double4* __restrict__ src, Pos3 pos,
+                                                      const double* __restrict__ zeta,
+                                                      const double* __restrict__ area,
+                                                      const int* __restrict__ smap, int ns, int ns_pad) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= ns_pad) return;
+  if (j < ns) {
+    const int t = smap[j];
+    const double circ = -zeta[t] * area[t];
+    const double q = FORM == FORM_DOT ? circ / (4 * kPi) : circ / (2 * kPi);
+    src[j] = make_double4(pos.at(t, 0), pos.at(t, 1), pos.at(t, 2), q);
+  } else {
+    src[j] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+
+// FP32C source records: one workgroup per tile computes the tile centre
+// (FP64 mean of its sources) and the tile's FP64 sum of q, then stores each
+// source as {fl32(y - c), fl32(q)} with q = (-zeta A)/(2 pi).  Padding
+// records sit 8 units away with q = 0.
+__global__ __launch_bounds__(kTile) void k_prep_f32c(float4* __restrict__ src, double4* __restrict__ ctr,
+                                                     double* __restrict__ srcq, Pos3 pos,
+                                                     const double* __restrict__ zeta,
+                                                     const double* __restrict__ area,
+                                                     const int* __restrict__ smap, int ns) {
+  __shared__ double red[4][kTile / 64];
+  const int t = blockIdx.x;
+  const int j = t * kTile + threadIdx.x;
+  const bool valid = j < ns;
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, q = 0.0;
+  if (valid) {
+    const int i = smap[j];
+    p0 = pos.at(i, 0);
+    p1 = pos.at(i, 1);
+    p2 = pos.at(i, 2);
+    q = (-zeta[i] * area[i]) / (2 * kPi);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double s0 = wave_sum(p0), s1 = wave_sum(p1), s2 = wave_sum(p2), sq = wave_sum(q);
+  if (lane == 0) {
+    red[0][wave] = s0;
+    red[1][wave] = s1;
+    red[2][wave] = s2;
+    red[3][wave] = sq;
+  }
+  __syncthreads();
+  const int nvalid = min(kTile, ns - t * kTile);
+  double c[4];
+  for (int k = 0; k < 4; ++k) c[k] = ((red[k][0] + red[k][1]) + (red[k][2] + red[k][3]));
+  const double inv = 1.0 / nvalid;
+  c[0] *= inv;
+  c[1] *= inv;
+  c[2] *= inv;
+  if (threadIdx.x == 0) ctr[t] = make_double4(c[0], c[1], c[2], c[3]);
+  src[j] = valid ? make_float4((float)(p0 - c[0]), (float)(p1 - c[1]), (float)(p2 - c[2]), (float)q)
+                 : make_float4(8.0f, 8.0f, 8.0f, 0.0f);
+  srcq[j] = q;
+}
+
+// Sum of per-tile q sums in tile order (one workgroup; deterministic).
+__global__ __launch_bounds__(kBlock) void k_qsum(const double4* __restrict__ ctr, int ntiles,
+                                                 double* __restrict__ out) {
+  __shared__ double red[kBlock / 64];
+  double s = 0.0;
+  for (int t = threadIdx.x; t < ntiles; t += kBlock) s += ctr[t].w;
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------
+// pair kernels
+// ---------------------------------------------------------------------------
+
+struct PairArgs {
+  Pos3 tgt;              // target positions
+  const int* self_slot;  // source slot of each target, or -1; may be null
+  int nt;                // targets
+  int ntiles;            // source tiles
+  int tiles_per_split;   // tiles per grid.y slice
+  double* part;          // [splits][kNC][nt_pad]
+  int nt_pad;
+};
+
+template <int R>
+__device__ __forceinline__ void load_targets(const PairArgs& a, double (&x)[R][3], int (&self)[R], int (&idx)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = (blockIdx.x * R + r) * kBlock + threadIdx.x;
+    idx[r] = i;
+    const int ii = min(i, a.nt - 1);
+    x[r][0] = a.tgt.at(ii, 0);
+    x[r][1] = a.tgt.at(ii, 1);
+    x[r][2] = a.tgt.at(ii, 2);
+    self[r] = (a.self_slot && i < a.nt) ? a.self_slot[ii] : -1;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ bool wave_has_self(const int (&self)[R], int base) {
+  int hit = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) hit |= (unsigned)(self[r] - base) < (unsigned)kTile;
+  return __ballot(hit) != 0;
+}
+
+template <int R>
+__device__ __forceinline__ void store_partials(const PairArgs& a, const int (&idx)[R], const double (&S)[R][kNC]) {
+  double* out = a.part + (long)blockIdx.y * kNC * a.nt_pad;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (idx[r] < a.nt) {
+#pragma unroll
+      for (int c = 0; c < kNC; ++c) out[(long)c * a.nt_pad + idx[r]] = S[r][c];
+    }
+  }
+}
+
+// One FP64 pair term; `me` drops the target's own source.
+template <int FORM, bool PSI, bool CHECK>
+__device__ __forceinline__ void pair64(const double (&x)[3], const double4 s, bool me, double (&S)[kNC]) {
+  if (FORM == FORM_DOT) {
+    double den = 1.0 - (x[0] * s.x + x[1] * s.y + x[2] * s.z);
+    double q = s.w;
+    if (CHECK && me) {
+      den = 1.0;
+      q = 0.0;
+    }
+    const double f = q * rcp64(den);
+    S[0] = fma(f, s.x, S[0]);
+    S[1] = fma(f, s.y, S[1]);
+    S[2] = fma(f, s.z, S[2]);
+    if (PSI) S[3] = fma(q, log(den), S[3]);
+  } else {
+    const double d0 = s.x - x[0], d1 = s.y - x[1], d2 = s.z - x[2];
+    double r2 = fma(d0, d0, fma(d1, d1, d2 * d2));
+    double q = s.w;
+    if (CHECK && me) {
+      r2 = 1.0;
+      q = 0.0;
+    }
+    const double f = q * rcp64(r2);
+    S[0] = fma(f, d0, S[0]);
+    S[1] = fma(f, d1, S[1]);
+    S[2] = fma(f, d2, S[2]);
+    if (PSI) S[3] = fma(q, log(0.5 * r2), S[3]);
+  }
+}
+
+template <int R, int FORM, bool PSI>
+__global__ __launch_bounds__(kBlock) void k_pair_f64(const double4* __restrict__ src, PairArgs a) {
+  double x[R][3];
+  int self[R], idx[R];
+  load_targets<R>(a, x, self, idx);
+  double S[R][kNC];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
+  const int t0 = blockIdx.y * a.tiles_per_split;
+  const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
+  for (int t = t0; t < t1; ++t) {
+    const int base = t * kTile;
+    const double4* ts = src + base;
+    if (!wave_has_self<R>(self, base)) {
+#pragma unroll 4
+      for (int j = 0; j < kTile; ++j) {
+        const double4 s = ts[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair64<FORM, PSI, false>(x[r], s, false, S[r]);
+      }
+    } else {
+      for (int j = 0; j < kTile; ++j) {
+        const double4 s = ts[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair64<FORM, PSI, true>(x[r], s, base + j == self[r], S[r]);
+      }
+    }
+  }
+  store_partials<R>(a, idx, S);
+}
+
+// One FP32 pair term from tile-local offsets; P accumulates q log2(|d|^2).
+template <bool PSI, bool CHECK>
+__device__ __forceinline__ void pair32(const float (&D)[3], const float4 s, bool me, float (&S)[kNC]) {
+  const float e0 = D[0] + s.x, e1 = D[1] + s.y, e2 = D[2] + s.z;
+  float r2 = fmaf(e0, e0, fmaf(e1, e1, e2 * e2));
+  float q = s.w;
+  if (CHECK && me) {
+    r2 = 1.0f;
+    q = 0.0f;
+  }
+  const float f = q * __builtin_amdgcn_rcpf(r2);
+  S[0] = fmaf(f, e0, S[0]);
+  S[1] = fmaf(f, e1, S[1]);
+  S[2] = fmaf(f, e2, S[2]);
+  if (PSI) S[3] = fmaf(q, __builtin_amdgcn_logf(r2), S[3]);
+}
+
+template <int R, bool PSI>
+__global__ __launch_bounds__(kBlock) void k_pair_f32c(const float4* __restrict__ src,
+                                                      const double4* __restrict__ ctr, PairArgs a) {
+  double x[R][3];
+  int self[R], idx[R];
+  load_targets<R>(a, x, self, idx);
+  double S[R][kNC];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
+  const int t0 = blockIdx.y * a.tiles_per_split;
+  const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
+  for (int t = t0; t < t1; ++t) {
+    const int base = t * kTile;
+    const double4 c = ctr[t];
+    float D[R][3], T[R][kNC];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      D[r][0] = (float)(c.x - x[r][0]);
+      D[r][1] = (float)(c.y - x[r][1]);
+      D[r][2] = (float)(c.z - x[r][2]);
+      for (int k = 0; k < kNC; ++k) T[r][k] = 0.0f;
+    }
+    const float4* ts = src + base;
+    if (!wave_has_self<R>(self, base)) {
+#pragma unroll 8
+      for (int j = 0; j < kTile; ++j) {
+        const float4 s = ts[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair32<PSI, false>(D[r], s, false, T[r]);
+      }
+    } else {
+      for (int j = 0; j < kTile; ++j) {
+        const float4 s = ts[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair32<PSI, true>(D[r], s, base + j == self[r], T[r]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
+  }
+  store_partials<R>(a, idx, S);
+}
+
+// ---------------------------------------------------------------------------
+// finalize: sum split partials in order, u = x x S, stream function
+// ---------------------------------------------------------------------------
+enum FinMode { FIN_DOT = 0, FIN_CHORD = 1, FIN_F32C = 2 };
+
+struct FinArgs {
+  const double* part;
+  int splits, nt, nt_pad;
+  Pos3 tgt;
+  MutPos3 u;             // velocity output
+  double* psi;           // may be null
+  long psi_stride;
+  const double* qtot;    // FP32C: total q over sources
+  const double* srcq;    // FP32C: FP64 q per source slot
+  const int* self_slot;  // may be null
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nt) return;
+  double S[kNC] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < a.splits; ++k)
+    for (int c = 0; c < kNC; ++c) S[c] += a.part[((long)k * kNC + c) * a.nt_pad + i];
+  const double x0 = a.tgt.at(i, 0), x1 = a.tgt.at(i, 1), x2 = a.tgt.at(i, 2);
+  a.u.at(i, 0) = x1 * S[2] - x2 * S[1];
+  a.u.at(i, 1) = x2 * S[0] - x0 * S[2];
+  a.u.at(i, 2) = x0 * S[1] - x1 * S[0];
+  if (a.psi) {
+    double p;
+    if (MODE == FIN_DOT) {
+      p = S[3];
+    } else if (MODE == FIN_CHORD) {
+      p = 0.5 * S[3];
+    } else {
+      // psi = sum_{j != i} (q_j / 2) (ln |d|^2 - ln 2)
+      double qx = *a.qtot;
+      const int s = a.self_slot ? a.self_slot[i] : -1;
+      if (s >= 0) qx -= a.srcq[s];
+      p = 0.5 * kLn2 * (S[3] - qx);
+    }
+    a.psi[i * a.psi_stride] = p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RK4 stage arithmetic (src/LpmBVERK4_Impl.hpp:11-121)
+// ---------------------------------------------------------------------------
+struct StageArgs {
+  int nt, nv, stage;  // stage 1..4
+  int lagr;
+  double dt, omega;
+  const double *u, *v, *w;  // velocity at the stage argument
+  double *x, *y, *z, *zeta;  // state (updated at stage 4)
+  const double* absv;        // conserved absolute vorticity (lagr)
+  double* kx[4];             // position increments, SoA triples [3][nt]
+  double* kz[4];             // vorticity increments (ref)
+  double *wx, *wy, *wz, *wzeta;  // next stage argument
+};
+
+__device__ __forceinline__ void normalize3(double& a, double& b, double& c) {
+  const double s = 1.0 / sqrt(a * a + b * b + c * c);
+  a *= s;
+  b *= s;
+  c *= s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_stage(StageArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nt) return;
+  const int s = a.stage - 1;
+  const long nt = a.nt;
+  // k_s = dt * u; dzeta_s = -2 Omega w dt (BVEVorticityTendency, src/LpmBVEKernels.hpp:317-319)
+  const double k0 = a.dt * a.u[i], k1 = a.dt * a.v[i], k2 = a.dt * a.w[i];
+  a.kx[s][i] = k0;
+  a.kx[s][nt + i] = k1;
+  a.kx[s][2 * nt + i] = k2;
+  const double kzs = -2.0 * a.omega * a.w[i] * a.dt;
+  if (!a.lagr) a.kz[s][i] = kzs;
+  if (a.stage < 4) {
+    const double c = a.stage == 3 ? 1.0 : 0.5;
+    double wx = a.x[i] + c * k0, wy = a.y[i] + c * k1, wz = a.z[i] + c * k2;
+    double wzeta;
+    if (a.lagr) {
+      normalize3(wx, wy, wz);
+      wzeta = a.absv[i] - 2.0 * a.omega * wz;
+    } else {
+      wzeta = a.zeta[i] + c * kzs;
+    }
+    a.wx[i] = wx;
+    a.wy[i] = wy;
+    a.wz[i] = wz;
+    a.wzeta[i] = wzeta;
+    return;
+  }
+  // BVERK4Update: x += (k1 + k4)/6 + (k2 + k3)/3
+  const double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
+  double* X[3] = {a.x, a.y, a.z};
+  double nx[3];
+  for (int c = 0; c < 3; ++c) {
+    const long o = c * nt + i;
+    nx[c] = X[c][i] + (sixth * (a.kx[0][o] + a.kx[3][o]) + third * (a.kx[1][o] + a.kx[2][o]));
+  }
+  double nz;
+  if (a.lagr) {
+    normalize3(nx[0], nx[1], nx[2]);
+    nz = a.absv[i] - 2.0 * a.omega * nx[2];
+  } else {
+    // faces combine vorticity stages (1, 2, 4, 4) (src/LpmBVERK4_Impl.hpp:120-121)
+    const double k3z = i < a.nv ? a.kz[2][i] : a.kz[3][i];
+    nz = a.zeta[i] + (sixth * (a.kz[0][i] + a.kz[3][i]) + third * (a.kz[1][i] + k3z));
+  }
+  for (int c = 0; c < 3; ++c) X[c][i] = nx[c];
+  a.zeta[i] = nz;
+}
+
+// absv = zeta + 2 Omega z
+__global__ __launch_bounds__(kBlock) void k_absv(int nt, double omega, const double* __restrict__ zeta,
+                                                 const double* __restrict__ z, double* __restrict__ absv) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nt) absv[i] = zeta[i] + 2.0 * omega * z[i];
+}
+
+// sum over sources of zeta * area, in slot order per lane then a fixed tree
+__global__ __launch_bounds__(kBlock) void k_circulation(const double* __restrict__ zeta,
+                                                        const double* __restrict__ area,
+                                                        const int* __restrict__ smap, int ns,
+                                                        double* __restrict__ out) {
+  __shared__ double red[kBlock / 64];
+  double s = 0.0;
+  for (int j = threadIdx.x; j < ns; j += kBlock) {
+    const int t = smap[j];
+    s += zeta[t] * area[t];
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Deterministic single-workgroup stream compaction of unmasked faces:
+// smap[slot] = offset + face, self_slot[face] = slot (or -1).
+__global__ __launch_bounds__(1024) void k_compact(const uint8_t* __restrict__ mask, int nf, int offset,
+                                                  int* __restrict__ smap, int* __restrict__ self_slot,
+                                                  int* __restrict__ count) {
+  __shared__ int cnt[1024];
+  const int chunk = (nf + 1023) / 1024;
+  const int lo = min(nf, (int)threadIdx.x * chunk), hi = min(nf, lo + chunk);
+  int c = 0;
+  for (int f = lo; f < hi; ++f) c += mask[f] == 0;
+  cnt[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < 1024; ++k) {
+      const int v = cnt[k];
+      cnt[k] = run;
+      run += v;
+    }
+    *count = run;
+  }
+  __syncthreads();
+  int slot = cnt[threadIdx.x];
+  for (int f = lo; f < hi; ++f) {
+    if (mask[f] == 0) {
+      smap[slot] = offset + f;
+      if (self_slot) self_slot[f] = slot;
+      ++slot;
+    } else if (self_slot) {
+      self_slot[f] = -1;
+    }
+  }
+}
+
+}  // namespace dev
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace {
+
+using namespace dev;
+
+#define LPM_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return fail(LPM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Launch shape of one evaluation.
+struct Shape {
+  int R = 2;
+  int splits = 1;
+  int ntiles = 0;
+  int tiles_per_split = 0;
+};
+
+Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
+  Shape sh;
+  sh.R = (R_req == 1 || R_req == 2 || R_req == 4) ? R_req : 2;
+  sh.ntiles = std::max(1, cdiv(ns, kTile));
+  const int wg_x = cdiv(nt, (long)kBlock * sh.R);
+  int splits = splits_req;
+  if (splits <= 0) {
+    // aim for >= 2048 workgroups (8 per CU on 256 CUs) with >= 4 tiles per split
+    splits = std::max(1, std::min(cdiv(2048, wg_x), std::max(1, sh.ntiles / 4)));
+  }
+  splits = std::min(splits, sh.ntiles);
+  sh.tiles_per_split = cdiv(sh.ntiles, splits);
+  sh.splits = cdiv(sh.ntiles, sh.tiles_per_split);
+  return sh;
+}
+
+// Device buffers of one evaluation: sources, partials, FP32C extras.
+struct EvalBuffers {
+  double4* src64 = nullptr;
+  float4* src32 = nullptr;
+  double4* ctr = nullptr;
+  double* srcq = nullptr;
+  double* qtot = nullptr;
+  double* part = nullptr;
+  size_t part_elems = 0;
+};
+
+// One source sweep over targets `tgt` from sources smap[ns] of the
+// target-indexed arrays (pos, zeta, area).  Writes u (and psi).
+struct EvalRequest {
+  Pos3 tgt;
+  int nt;
+  Pos3 srcpos;
+  const double* zeta;
+  const double* area;
+  const int* smap;
+  int ns;
+  const int* self_slot;
+  MutPos3 u;
+  double* psi;
+  long psi_stride;
+  int prec;  // lpm_prec
+  int form;  // Form (FP64 only)
+  bool want_psi;
+  Shape shape;
+};
+
+template <int R>
+void launch_pair(const EvalRequest& q, const EvalBuffers& b, const PairArgs& pa, dim3 grid, hipStream_t st) {
+  if (q.prec == LPM_PREC_FP32C) {
+    if (q.want_psi)
+      hipLaunchKernelGGL((k_pair_f32c<R, true>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, pa);
+    else
+      hipLaunchKernelGGL((k_pair_f32c<R, false>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, pa);
+  } else if (q.form == FORM_DOT) {
+    if (q.want_psi)
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_DOT, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
+    else
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_DOT, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
+  } else {
+    if (q.want_psi)
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_CHORD, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
+    else
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_CHORD, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
+  }
+}
+
+// Queue prep + pair + finalize on `st`.  ev0/ev1 (optional) bracket the pair
+// kernel.
+int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  const Shape& sh = q.shape;
+  const int ns_pad = sh.ntiles * kTile;
+  if (q.prec == LPM_PREC_FP32C) {
+    hipLaunchKernelGGL(k_prep_f32c, dim3(sh.ntiles), dim3(kTile), 0, st, b.src32, b.ctr, b.srcq, q.srcpos, q.zeta,
+                       q.area, q.smap, q.ns);
+    if (q.want_psi) hipLaunchKernelGGL(k_qsum, dim3(1), dim3(kBlock), 0, st, b.ctr, sh.ntiles, b.qtot);
+  } else if (q.form == FORM_DOT) {
+    hipLaunchKernelGGL((k_prep_f64<FORM_DOT>), dim3(cdiv(ns_pad, kBlock)), dim3(kBlock), 0, st, b.src64, q.srcpos,
+                       q.zeta, q.area, q.smap, q.ns, ns_pad);
+  } else {
+    hipLaunchKernelGGL((k_prep_f64<FORM_CHORD>), dim3(cdiv(ns_pad, kBlock)), dim3(kBlock), 0, st, b.src64,
+                       q.srcpos, q.zeta, q.area, q.smap, q.ns, ns_pad);
+  }
+  LPM_HIP(hipGetLastError());
+  PairArgs pa;
+  pa.tgt = q.tgt;
+  pa.self_slot = q.self_slot;
+  pa.nt = q.nt;
+  pa.ntiles = sh.ntiles;
+  pa.tiles_per_split = sh.tiles_per_split;
+  pa.part = b.part;
+  pa.nt_pad = q.nt;
+  const dim3 grid(cdiv(q.nt, (long)kBlock * sh.R), sh.splits);
+  if (ev0) LPM_HIP(hipEventRecord(ev0, st));
+  switch (sh.R) {
+    case 1: launch_pair<1>(q, b, pa, grid, st); break;
+    case 4: launch_pair<4>(q, b, pa, grid, st); break;
+    default: launch_pair<2>(q, b, pa, grid, st); break;
+  }
+  LPM_HIP(hipGetLastError());
+  if (ev1) LPM_HIP(hipEventRecord(ev1, st));
+  FinArgs fa;
+  fa.part = b.part;
+  fa.splits = sh.splits;
+  fa.nt = q.nt;
+  fa.nt_pad = q.nt;
+  fa.tgt = q.tgt;
+  fa.u = q.u;
+  fa.psi = q.want_psi ? q.psi : nullptr;
+  fa.psi_stride = q.psi_stride;
+  fa.qtot = b.qtot;
+  fa.srcq = b.srcq;
+  fa.self_slot = q.self_slot;
+  const dim3 fg(cdiv(q.nt, kBlock));
+  if (q.prec == LPM_PREC_FP32C)
+    hipLaunchKernelGGL((k_finalize<FIN_F32C>), fg, dim3(kBlock), 0, st, fa);
+  else if (q.form == FORM_DOT)
+    hipLaunchKernelGGL((k_finalize<FIN_DOT>), fg, dim3(kBlock), 0, st, fa);
+  else
+    hipLaunchKernelGGL((k_finalize<FIN_CHORD>), fg, dim3(kBlock), 0, st, fa);
+  LPM_HIP(hipGetLastError());
+  return LPM_OK;
+}
+
+// Sizes (bytes) of the evaluation buffers for ns sources, nt targets.
+struct BufferPlan {
+  size_t src64, src32, ctr, srcq, part;
+};
+BufferPlan plan_buffers(int nt, const Shape& sh) {
+  const size_t ns_pad = (size_t)sh.ntiles * kTile;
+  BufferPlan p;
+  p.src64 = ns_pad * sizeof(double4);
+  p.src32 = ns_pad * sizeof(float4);
+  p.ctr = (size_t)sh.ntiles * sizeof(double4) + 256;
+  p.srcq = ns_pad * sizeof(double);
+  p.part = (size_t)sh.splits * kNC * nt * sizeof(double);
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// per-device workspace for the functor entry points
+// ---------------------------------------------------------------------------
+struct Workspace {
+  std::mutex mu;
+  char* base = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<int, Workspace*> g_ws;
+
+Workspace* workspace_for(int dev) {
+  std::lock_guard<std::mutex> g(g_ws_mu);
+  auto& w = g_ws[dev];
+  if (!w) w = new Workspace();
+  return w;
+}
+
+inline size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// Functor evaluation on caller-owned device arrays: compact the mask, then
+// one FP64 dot-form evaluation.
+int functor_eval(double* u, double* psi, const double* tx, int64_t nt, const double* fx, const double* fzeta,
+                 const double* farea, const uint8_t* fmask, int64_t nf, bool collocated, void* stream) {
+  if (nt < 0 || nf < 0 || nt > INT32_MAX / 4 || nf > INT32_MAX / 4) return fail(LPM_ERR_ARG, "functor: bad sizes");
+  if (nt == 0) return ok();
+  if (!tx || !fx || !fzeta || !farea || !fmask || (!u && !psi)) return fail(LPM_ERR_ARG, "functor: null pointer");
+  int dev = 0;
+  LPM_HIP(hipGetDevice(&dev));
+  hipStream_t st = (hipStream_t)stream;
+  Workspace* ws = workspace_for(dev);
+  std::lock_guard<std::mutex> g(ws->mu);
+  // the number of active sources is only known on the device: plan for nf
+  const Shape sh = choose_shape((int)nt, (int)std::max<int64_t>(nf, 1), 2, 0);
+  const BufferPlan bp = plan_buffers((int)nt, sh);
+  const size_t need = align256(bp.src64) + align256(bp.part) + align256(sizeof(int) * (nf + 1)) +
+                      align256(sizeof(int) * (nf + 1)) + align256(sizeof(int)) +
+                      align256(sizeof(double) * 3 * (size_t)nt) + 4096;
+  if (ws->bytes < need) {
+    if (ws->base) {
+      LPM_HIP(hipStreamSynchronize(st));
+      LPM_HIP(hipFree(ws->base));
+      ws->base = nullptr;
+      ws->bytes = 0;
+    }
+    if (hipMalloc(&ws->base, need) != hipSuccess) return fail(LPM_ERR_NOMEM, "functor: workspace allocation failed");
+    ws->bytes = need;
+  }
+  char* p = ws->base;
+  auto carve = [&](size_t n) {
+    char* r = p;
+    p += align256(n);
+    return r;
+  };
+  EvalBuffers b;
+  b.src64 = (double4*)carve(bp.src64);
+  b.part = (double*)carve(bp.part);
+  int* smap = (int*)carve(sizeof(int) * (nf + 1));
+  int* self = (int*)carve(sizeof(int) * (nf + 1));
+  int* count = (int*)carve(sizeof(int));
+  double* utmp = (double*)carve(sizeof(double) * 3 * (size_t)nt);
+  hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, st, fmask, (int)nf, 0, smap, collocated ? self : nullptr,
+                     count);
+  LPM_HIP(hipGetLastError());
+  int ns = 0;
+  LPM_HIP(hipMemcpyAsync(&ns, count, sizeof(int), hipMemcpyDeviceToHost, st));
+  LPM_HIP(hipStreamSynchronize(st));
+  EvalRequest q;
+  q.tgt = Pos3{tx, 3, 1};
+  q.nt = (int)nt;
+  q.srcpos = Pos3{fx, 3, 1};
+  q.zeta = fzeta;
+  q.area = farea;
+  q.smap = smap;
+  q.ns = ns;
+  q.self_slot = collocated ? self : nullptr;
+  q.u = MutPos3{u ? u : utmp, 3, 1};
+  q.psi = psi;
+  q.psi_stride = 1;
+  q.prec = LPM_PREC_FP64;
+  q.form = FORM_DOT;
+  q.want_psi = psi != nullptr;
+  q.shape = choose_shape((int)nt, std::max(ns, 1), 2, 0);
+  if (q.shape.splits > sh.splits || q.shape.ntiles > sh.ntiles) q.shape = sh;
+  if (ns == 0) {
+    // no active source: every sum is zero
+    if (u) LPM_HIP(hipMemsetAsync(u, 0, sizeof(double) * 3 * nt, st));
+    if (psi) LPM_HIP(hipMemsetAsync(psi, 0, sizeof(double) * nt, st));
+    return ok();
+  }
+  int rc = enqueue_eval(q, b, st, nullptr, nullptr);
+  if (rc) return rc;
+  // the workspace is reused by the next call: finish before releasing it
+  LPM_HIP(hipStreamSynchronize(st));
+  return ok();
+}
+
+}  // namespace
+}  // namespace lpm
+
+// ===========================================================================
+// solver context
+// ===========================================================================
+struct lpm_bve_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t nv = 0, nf = 0;
+  int nt = 0, ns = 0;  // targets = vertices then active faces; sources = active faces
+  // tree-face index of each active face; host copies of every face's
+  // position and vorticity (divided faces are not evaluated and keep theirs)
+  std::vector<int64_t> fmap;
+  std::vector<double> face_x_host, face_zeta_host;
+  // device arrays (SoA, target-indexed: vertices then tree faces)
+  double *x = nullptr, *y = nullptr, *z = nullptr, *zeta = nullptr;
+  double *u = nullptr, *v = nullptr, *w = nullptr, *psi = nullptr;
+  double *area = nullptr, *absv = nullptr;
+  double *wx = nullptr, *wy = nullptr, *wz = nullptr, *wzeta = nullptr;
+  double* kx[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* kz[4] = {nullptr, nullptr, nullptr, nullptr};
+  int *smap = nullptr, *self_slot = nullptr;
+  double* circ = nullptr;
+  lpm::EvalBuffers eb;
+  std::vector<void*> allocations;
+  // launch shape
+  int R_req = 2, splits_req = 0;
+  // velocity validity: form and precision that produced u
+  bool vel_valid = false;
+  int vel_prec = -1, vel_form = -1;
+  bool psi_valid = false;
+  // absv is valid for this omega
+  bool absv_valid = false;
+  double absv_omega = 0.0;
+  // captured step graphs
+  struct GraphKey {
+    double dt, omega;
+    int scheme, prec, R, splits;
+    bool operator<(const GraphKey& o) const {
+      return std::tie(dt, omega, scheme, prec, R, splits) < std::tie(o.dt, o.omega, o.scheme, o.prec, o.R, o.splits);
+    }
+  };
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  // pair-kernel timing
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timing_events;
+  bool timing = false;
+};
+
+namespace lpm {
+namespace {
+
+void free_ctx_memory(lpm_bve_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  for (void* p : c->allocations) (void)hipFree(p);
+  c->allocations.clear();
+  for (auto& e : c->timing_events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  c->timing_events.clear();
+}
+
+Shape ctx_shape(const lpm_bve_ctx* c) { return choose_shape(c->nt, std::max(c->ns, 1), c->R_req, c->splits_req); }
+
+// Make sure the evaluation buffers fit the current launch shape.
+int ensure_eval_buffers(lpm_bve_ctx* c, const Shape& sh) {
+  const BufferPlan bp = plan_buffers(c->nt, sh);
+  const size_t part_elems = bp.part / sizeof(double);
+  if (c->eb.src64 && c->eb.part_elems >= part_elems) return LPM_OK;
+  if (c->eb.part) {
+    // grow the partial buffer only
+    LPM_HIP(hipStreamSynchronize(c->stream));
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    auto it = std::find(c->allocations.begin(), c->allocations.end(), (void*)c->eb.part);
+    if (it != c->allocations.end()) c->allocations.erase(it);
+    LPM_HIP(hipFree(c->eb.part));
+    c->eb.part = nullptr;
+    if (hipMalloc(&c->eb.part, bp.part) != hipSuccess) return fail(LPM_ERR_NOMEM, "partial buffer allocation failed");
+    c->allocations.push_back(c->eb.part);
+    c->eb.part_elems = part_elems;
+    return LPM_OK;
+  }
+  auto alloc = [&](void** p, size_t n) -> int {
+    if (hipMalloc(p, std::max<size_t>(n, 256)) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
+    c->allocations.push_back(*p);
+    return LPM_OK;
+  };
+  int rc;
+  if ((rc = alloc((void**)&c->eb.src64, bp.src64))) return rc;
+  if ((rc = alloc((void**)&c->eb.src32, bp.src32))) return rc;
+  if ((rc = alloc((void**)&c->eb.ctr, bp.ctr))) return rc;
+  if ((rc = alloc((void**)&c->eb.srcq, bp.srcq))) return rc;
+  if ((rc = alloc((void**)&c->eb.qtot, 256))) return rc;
+  if ((rc = alloc((void**)&c->eb.part, bp.part))) return rc;
+  c->eb.part_elems = part_elems;
+  return LPM_OK;
+}
+
+// Evaluate velocity (and psi) at positions (px, py, pz) with vorticity pz.
+int ctx_eval(lpm_bve_ctx* c, const double* px, const double* pzeta, int prec, int form, bool want_psi) {
+  const Shape sh = ctx_shape(c);
+  int rc = ensure_eval_buffers(c, sh);
+  if (rc) return rc;
+  EvalRequest q;
+  q.tgt = Pos3{px, 1, (long)c->nt};
+  q.nt = c->nt;
+  q.srcpos = q.tgt;
+  q.zeta = pzeta;
+  q.area = c->area;
+  q.smap = c->smap;
+  q.ns = c->ns;
+  q.self_slot = c->self_slot;
+  q.u = MutPos3{c->u, 1, (long)c->nt};
+  q.psi = c->psi;
+  q.psi_stride = 1;
+  q.prec = prec;
+  q.form = form;
+  q.want_psi = want_psi;
+  q.shape = sh;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    LPM_HIP(hipEventCreate(&e0));
+    LPM_HIP(hipEventCreate(&e1));
+    c->timing_events.push_back({e0, e1});
+  }
+  return enqueue_eval(q, c->eb, c->stream, e0, e1);
+}
+
+int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme) {
+  StageArgs a;
+  a.nt = c->nt;
+  a.nv = (int)c->nv;
+  a.stage = stage;
+  a.lagr = scheme == LPM_SCHEME_LAGR;
+  a.dt = dt;
+  a.omega = omega;
+  a.u = c->u;
+  a.v = c->v;
+  a.w = c->w;
+  a.x = c->x;
+  a.y = c->y;
+  a.z = c->z;
+  a.zeta = c->zeta;
+  a.absv = c->absv;
+  for (int s = 0; s < 4; ++s) {
+    a.kx[s] = c->kx[s];
+    a.kz[s] = c->kz[s];
+  }
+  a.wx = c->wx;
+  a.wy = c->wy;
+  a.wz = c->wz;
+  a.wzeta = c->wzeta;
+  hipLaunchKernelGGL(k_stage, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, c->stream, a);
+  LPM_HIP(hipGetLastError());
+  return LPM_OK;
+}
+
+// One advance_timestep (src/LpmBVERK4_Impl.hpp:42-130) queued on the stream.
+int ctx_enqueue_step(lpm_bve_ctx* c, double dt, double omega, int scheme, int prec) {
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  int rc;
+  for (int stage = 1; stage <= 3; ++stage) {
+    if ((rc = ctx_stage(c, stage, dt, omega, scheme))) return rc;
+    if ((rc = ctx_eval(c, c->wx, c->wzeta, prec, form, false))) return rc;
+  }
+  if ((rc = ctx_stage(c, 4, dt, omega, scheme))) return rc;
+  return ctx_eval(c, c->x, c->zeta, prec, form, false);
+}
+
+}  // namespace
+}  // namespace lpm
+
+using namespace lpm;
+
+extern "C" {
+
+int lpm_device_count(int* count) {
+  LPM_HIP(hipGetDeviceCount(count));
+  return ok();
+}
+
+int lpm_bve_vertex_velocity(double* u, const double* vx, int64_t nv, const double* fx, const double* fzeta,
+                            const double* farea, const uint8_t* fmask, int64_t nf, void* stream) {
+  return functor_eval(u, nullptr, vx, nv, fx, fzeta, farea, fmask, nf, false, stream);
+}
+int lpm_bve_face_velocity(double* u, const double* fx, const double* fzeta, const double* farea,
+                          const uint8_t* fmask, int64_t nf, void* stream) {
+  return functor_eval(u, nullptr, fx, nf, fx, fzeta, farea, fmask, nf, true, stream);
+}
+int lpm_bve_vertex_streamfn(double* psi, const double* vx, int64_t nv, const double* fx, const double* fzeta,
+                            const double* farea, const uint8_t* fmask, int64_t nf, void* stream) {
+  return functor_eval(nullptr, psi, vx, nv, fx, fzeta, farea, fmask, nf, false, stream);
+}
+int lpm_bve_face_streamfn(double* psi, const double* fx, const double* fzeta, const double* farea,
+                          const uint8_t* fmask, int64_t nf, void* stream) {
+  return functor_eval(nullptr, psi, fx, nf, fx, fzeta, farea, fmask, nf, true, stream);
+}
+
+int lpm_bve_create(int device, lpm_bve_ctx** out) {
+  if (!out) return fail(LPM_ERR_ARG, "lpm_bve_create: null output");
+  int n = 0;
+  LPM_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(LPM_ERR_ARG, "lpm_bve_create: no such device");
+  LPM_HIP(hipSetDevice(device));
+  lpm_bve_ctx* c = new lpm_bve_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(LPM_ERR_HIP, "lpm_bve_create: stream creation failed");
+  }
+  *out = c;
+  return ok();
+}
+
+int lpm_bve_destroy(lpm_bve_ctx* c) {
+  if (!c) return ok();
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_ctx_memory(c);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ok();
+}
+
+int lpm_bve_set_state(lpm_bve_ctx* c, int64_t nv, const double* vx, const double* vzeta, int64_t nf,
+                      const double* fx, const double* fzeta, const double* farea, const uint8_t* fmask) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_state: null context");
+  if (nv < 0 || nf < 1 || nv + nf > (int64_t)INT32_MAX / 8)
+    return fail(LPM_ERR_ARG, "lpm_bve_set_state: sizes out of range");
+  if ((nv && (!vx || !vzeta)) || !fx || !fzeta || !farea || !fmask)
+    return fail(LPM_ERR_ARG, "lpm_bve_set_state: null array");
+  LPM_HIP(hipSetDevice(c->device));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  free_ctx_memory(c);
+  c->eb = EvalBuffers();
+  c->nv = nv;
+  c->nf = nf;
+  // Divided faces are sources of nothing (mask) and, as targets, sit on
+  // their centre child's centroid, where the reference's collocated sum
+  // divides by 1 - x.y = 0; they are kept on the host and not evaluated.
+  c->fmap.clear();
+  for (int64_t f = 0; f < nf; ++f)
+    if (!fmask[f]) c->fmap.push_back(f);
+  c->ns = (int)c->fmap.size();
+  if (c->ns == 0) return fail(LPM_ERR_ARG, "lpm_bve_set_state: no active (unmasked) face");
+  c->face_x_host.assign(fx, fx + 3 * nf);
+  c->face_zeta_host.assign(fzeta, fzeta + nf);
+  c->nt = (int)(nv + c->ns);
+  const int nt = c->nt;
+  // host staging: SoA, target-indexed (vertices, then active faces in tree order)
+  std::vector<double> h(4 * (size_t)nt), harea(nt, 0.0);
+  std::vector<int> hsmap(c->ns), hself(nt, -1);
+  for (int64_t i = 0; i < nv; ++i) {
+    for (int k = 0; k < 3; ++k) h[k * (size_t)nt + i] = vx[3 * i + k];
+    h[3 * (size_t)nt + i] = vzeta[i];
+  }
+  for (int j = 0; j < c->ns; ++j) {
+    const int64_t f = c->fmap[j], i = nv + j;
+    for (int k = 0; k < 3; ++k) h[k * (size_t)nt + i] = fx[3 * f + k];
+    h[3 * (size_t)nt + i] = fzeta[f];
+    harea[i] = farea[f];
+    hsmap[j] = (int)i;
+    hself[i] = j;
+  }
+  auto alloc = [&](void** p, size_t n) -> int {
+    if (hipMalloc(p, std::max<size_t>(n, 256)) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
+    c->allocations.push_back(*p);
+    return LPM_OK;
+  };
+  const size_t d = sizeof(double) * (size_t)nt;
+  int rc = 0;
+  // position, stage-argument and velocity triples are contiguous [3][nt]
+  // blocks so kernels address them as one SoA array
+  double** triples[] = {&c->x, &c->wx, &c->u};
+  for (double** p : triples)
+    if ((rc = alloc((void**)p, 3 * d))) return rc;
+  c->y = c->x + nt;
+  c->z = c->x + 2 * (size_t)nt;
+  c->wy = c->wx + nt;
+  c->wz = c->wx + 2 * (size_t)nt;
+  c->v = c->u + nt;
+  c->w = c->u + 2 * (size_t)nt;
+  double** singles[] = {&c->zeta, &c->psi, &c->area, &c->absv, &c->wzeta};
+  for (double** p : singles)
+    if ((rc = alloc((void**)p, d))) return rc;
+  for (int s = 0; s < 4; ++s) {
+    if ((rc = alloc((void**)&c->kx[s], 3 * d))) return rc;
+    if ((rc = alloc((void**)&c->kz[s], d))) return rc;
+  }
+  if ((rc = alloc((void**)&c->smap, sizeof(int) * c->ns))) return rc;
+  if ((rc = alloc((void**)&c->self_slot, sizeof(int) * nt))) return rc;
+  if ((rc = alloc((void**)&c->circ, 256))) return rc;
+  LPM_HIP(hipMemcpyAsync(c->x, h.data(), d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->y, h.data() + nt, d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->z, h.data() + 2 * (size_t)nt, d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->zeta, h.data() + 3 * (size_t)nt, d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->area, harea.data(), d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->smap, hsmap.data(), sizeof(int) * c->ns, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->self_slot, hself.data(), sizeof(int) * nt, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  c->vel_valid = false;
+  c->psi_valid = false;
+  c->absv_valid = false;
+  return ok();
+}
+
+int lpm_bve_set_velocity(lpm_bve_ctx* c, const double* vu, const double* fu) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_set_velocity: no state");
+  if ((c->nv && !vu) || !fu) return fail(LPM_ERR_ARG, "lpm_bve_set_velocity: null array");
+  LPM_HIP(hipSetDevice(c->device));
+  const int nt = c->nt;
+  std::vector<double> h(3 * (size_t)nt);
+  for (int64_t i = 0; i < c->nv; ++i)
+    for (int k = 0; k < 3; ++k) h[k * (size_t)nt + i] = vu[3 * i + k];
+  for (int j = 0; j < c->ns; ++j)
+    for (int k = 0; k < 3; ++k) h[k * (size_t)nt + c->nv + j] = fu[3 * c->fmap[j] + k];
+  const size_t d = sizeof(double) * nt;
+  LPM_HIP(hipMemcpyAsync(c->u, h.data(), d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->v, h.data() + nt, d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->w, h.data() + 2 * (size_t)nt, d, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  c->vel_valid = true;
+  c->vel_prec = -1;  // caller-provided: accepted for any scheme
+  c->vel_form = -1;
+  return ok();
+}
+
+int lpm_bve_solve(lpm_bve_ctx* c, int prec, int scheme, int want_psi) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_solve: no state");
+  if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32C) return fail(LPM_ERR_ARG, "lpm_bve_solve: bad precision");
+  if (scheme != LPM_SCHEME_REF && scheme != LPM_SCHEME_LAGR) return fail(LPM_ERR_ARG, "lpm_bve_solve: bad scheme");
+  LPM_HIP(hipSetDevice(c->device));
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  int rc = ctx_eval(c, c->x, c->zeta, prec, form, want_psi != 0);
+  if (rc) return rc;
+  c->vel_valid = true;
+  c->vel_prec = prec;
+  c->vel_form = form;
+  c->psi_valid = want_psi != 0;
+  return ok();
+}
+
+int lpm_bve_advance(lpm_bve_ctx* c, double dt, double omega, int scheme, int prec, int nsteps, int use_graph) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_advance: no state");
+  if (scheme != LPM_SCHEME_REF && scheme != LPM_SCHEME_LAGR) return fail(LPM_ERR_ARG, "lpm_bve_advance: bad scheme");
+  if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32C) return fail(LPM_ERR_ARG, "lpm_bve_advance: bad precision");
+  if (prec == LPM_PREC_FP32C && scheme != LPM_SCHEME_LAGR)
+    return fail(LPM_ERR_ARG, "lpm_bve_advance: FP32C pair sums require the renormalised (LAGR) scheme");
+  if (nsteps < 0) return fail(LPM_ERR_ARG, "lpm_bve_advance: nsteps < 0");
+  if (!(dt == dt) || !(omega == omega)) return fail(LPM_ERR_ARG, "lpm_bve_advance: dt/omega is NaN");
+  LPM_HIP(hipSetDevice(c->device));
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  int rc;
+  if (scheme == LPM_SCHEME_LAGR && !(c->absv_valid && c->absv_omega == omega)) {
+    hipLaunchKernelGGL(k_absv, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, c->stream, c->nt, omega, c->zeta, c->z,
+                       c->absv);
+    LPM_HIP(hipGetLastError());
+    c->absv_valid = true;
+    c->absv_omega = omega;
+  }
+  if (!c->vel_valid || (c->vel_prec >= 0 && (c->vel_prec != prec || c->vel_form != form))) {
+    if ((rc = ctx_eval(c, c->x, c->zeta, prec, form, false))) return rc;
+    c->vel_valid = true;
+    c->vel_prec = prec;
+    c->vel_form = form;
+  }
+  const Shape sh = ctx_shape(c);
+  if ((rc = ensure_eval_buffers(c, sh))) return rc;
+  if (use_graph && !c->timing && nsteps > 0) {
+    lpm_bve_ctx::GraphKey key{dt, omega, scheme, prec, sh.R, sh.splits};
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+      hipGraph_t g;
+      LPM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = ctx_enqueue_step(c, dt, omega, scheme, prec);
+      hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (rc) return rc;
+      if (ce != hipSuccess) return fail(LPM_ERR_HIP, std::string("step capture failed: ") + hipGetErrorString(ce));
+      hipGraphExec_t ge;
+      hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(LPM_ERR_HIP, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+      it = c->graphs.emplace(key, ge).first;
+    }
+    for (int s = 0; s < nsteps; ++s) LPM_HIP(hipGraphLaunch(it->second, c->stream));
+  } else {
+    for (int s = 0; s < nsteps; ++s)
+      if ((rc = ctx_enqueue_step(c, dt, omega, scheme, prec))) return rc;
+  }
+  c->vel_prec = prec;
+  c->vel_form = form;
+  c->psi_valid = false;
+  return ok();
+}
+
+int lpm_bve_get_state(lpm_bve_ctx* c, double* vx, double* vzeta, double* vu, double* vpsi, double* fx,
+                      double* fzeta, double* fu, double* fpsi) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_get_state: no state");
+  LPM_HIP(hipSetDevice(c->device));
+  const int nt = c->nt;
+  const size_t d = sizeof(double) * nt;
+  std::vector<double> h(8 * (size_t)nt);
+  double* srcs[8] = {c->x, c->y, c->z, c->zeta, c->u, c->v, c->w, c->psi};
+  for (int k = 0; k < 8; ++k) LPM_HIP(hipMemcpyAsync(h.data() + k * (size_t)nt, srcs[k], d, hipMemcpyDeviceToHost, c->stream));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  auto col = [&](int k, int64_t i) { return h[k * (size_t)nt + i]; };
+  for (int64_t i = 0; i < c->nv; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      if (vx) vx[3 * i + k] = col(k, i);
+      if (vu) vu[3 * i + k] = col(4 + k, i);
+    }
+    if (vzeta) vzeta[i] = col(3, i);
+    if (vpsi) vpsi[i] = col(7, i);
+  }
+  // divided faces: stored position and vorticity, zero velocity and psi
+  if (fx) std::memcpy(fx, c->face_x_host.data(), sizeof(double) * 3 * c->nf);
+  if (fzeta) std::memcpy(fzeta, c->face_zeta_host.data(), sizeof(double) * c->nf);
+  if (fu) std::memset(fu, 0, sizeof(double) * 3 * c->nf);
+  if (fpsi) std::memset(fpsi, 0, sizeof(double) * c->nf);
+  for (int j = 0; j < c->ns; ++j) {
+    const int64_t f = c->fmap[j], i = c->nv + j;
+    for (int k = 0; k < 3; ++k) {
+      if (fx) fx[3 * f + k] = col(k, i);
+      if (fu) fu[3 * f + k] = col(4 + k, i);
+    }
+    if (fzeta) fzeta[f] = col(3, i);
+    if (fpsi) fpsi[f] = col(7, i);
+  }
+  return ok();
+}
+
+int lpm_bve_circulation(lpm_bve_ctx* c, double* circ) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_circulation: no state");
+  if (!circ) return fail(LPM_ERR_ARG, "lpm_bve_circulation: null output");
+  LPM_HIP(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_circulation, dim3(1), dim3(kBlock), 0, c->stream, c->zeta, c->area, c->smap, c->ns, c->circ);
+  LPM_HIP(hipGetLastError());
+  LPM_HIP(hipMemcpyAsync(circ, c->circ, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  return ok();
+}
+
+int lpm_bve_synchronize(lpm_bve_ctx* c) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_synchronize: null context");
+  LPM_HIP(hipSetDevice(c->device));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  return ok();
+}
+
+int lpm_bve_sizes(lpm_bve_ctx* c, int64_t* s) {
+  if (!c || !s) return fail(LPM_ERR_ARG, "lpm_bve_sizes: null argument");
+  s[0] = c->nv;
+  s[1] = c->nf;
+  s[2] = c->ns;
+  s[3] = c->nt;
+  return ok();
+}
+
+int lpm_bve_set_launch(lpm_bve_ctx* c, int targets_per_lane, int source_splits) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_launch: null context");
+  if (targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4)
+    return fail(LPM_ERR_ARG, "lpm_bve_set_launch: targets_per_lane must be 1, 2 or 4");
+  if (source_splits < 0) return fail(LPM_ERR_ARG, "lpm_bve_set_launch: source_splits < 0");
+  c->R_req = targets_per_lane;
+  c->splits_req = source_splits;
+  return ok();
+}
+
+int lpm_bve_stream(lpm_bve_ctx* c, void** stream, int* device) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_stream: null context");
+  if (stream) *stream = (void*)c->stream;
+  if (device) *device = c->device;
+  return ok();
+}
+
+int lpm_bve_last_pair_timing(lpm_bve_ctx* c, int* launches, double* mean_ms) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_last_pair_timing: null context");
+  LPM_HIP(hipSetDevice(c->device));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  double total = 0.0;
+  int n = 0;
+  for (auto& e : c->timing_events) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+      total += ms;
+      ++n;
+    }
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  c->timing_events.clear();
+  if (launches) *launches = n;
+  if (mean_ms) *mean_ms = n ? total / n : 0.0;
+  return ok();
+}
+
+// Enable (1) or disable (0) per-launch timing events around pair kernels.
+// Timed steps run without graph replay.
+int lpm_bve_enable_timing(lpm_bve_ctx* c, int on) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_enable_timing: null context");
+  c->timing = on != 0;
+  return ok();
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// C ABI for the host-side pieces: mesh build, initial conditions, errors.
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "lpm_bve.h"
+#include "lpm_ic.hpp"
+#include "lpm_mesh.hpp"
+#include "lpm_status.hpp"
+
+namespace lpm {
+namespace {
+thread_local std::string g_last_error;
+}
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+int ok() {
+  g_last_error.clear();
+  return LPM_OK;
+}
+}  // namespace lpm
+
+using namespace lpm;
+
+extern "C" {
+
+const char* lpm_last_error(void) { return g_last_error.c_str(); }
+
+int lpm_version(void) { return 10000; }
+
+int lpm_icos_alloc(int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces) {
+  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_icos_alloc: level must be in [0, 12]");
+  int64_t v, e, f;
+  icos_max_allocations(level, v, e, f);
+  if (nverts) *nverts = v;
+  if (nedges) *nedges = e;
+  if (nfaces) *nfaces = f;
+  return ok();
+}
+
+int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vlag, double* fx, double* flag,
+                       double* farea, uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts) {
+  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_icos_tree_init: level must be in [0, 12]");
+  try {
+    SeedTables file_seed;
+    const SeedTables* seed = &builtin_icos_seed();
+    if (seed_path) {
+      std::string err;
+      if (!read_icos_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, "lpm_icos_tree_init: " + err);
+      seed = &file_seed;
+    }
+    const TreeMesh m = build_icos_tree(level, *seed);
+    const int64_t nv = m.nverts(), nf = m.nfaces();
+    if (vx) std::memcpy(vx, m.vert_phys.data(), sizeof(double) * 3 * nv);
+    if (vlag) std::memcpy(vlag, m.vert_lag.data(), sizeof(double) * 3 * nv);
+    if (fx) std::memcpy(fx, m.face_phys.data(), sizeof(double) * 3 * nf);
+    if (flag) std::memcpy(flag, m.face_lag.data(), sizeof(double) * 3 * nf);
+    if (farea) std::memcpy(farea, m.face_area.data(), sizeof(double) * nf);
+    if (fmask) std::memcpy(fmask, m.face_mask.data(), nf);
+    if (fverts)
+      for (int64_t i = 0; i < 3 * nf; ++i) fverts[i] = m.face_verts[i];
+    if (fparent)
+      for (int64_t i = 0; i < nf; ++i) fparent[i] = m.face_parent[i];
+    if (counts) {
+      counts[0] = nv;
+      counts[1] = m.n_edges;
+      counts[2] = nf;
+      counts[3] = m.nleaves();
+    }
+  } catch (const std::exception& e) {
+    return fail(LPM_ERR_NOMEM, std::string("lpm_icos_tree_init: ") + e.what());
+  }
+  return ok();
+}
+
+int lpm_ic_gauss_params(uint64_t seed, int nvort, double amp_scale, double* centers, double* amps) {
+  if (nvort < 0) return fail(LPM_ERR_ARG, "lpm_ic_gauss_params: nvort < 0");
+  CompositeVorticity ic;
+  ic.draw_vortices(seed, nvort, amp_scale, 1.0);
+  for (int k = 0; k < nvort; ++k) {
+    for (int j = 0; j < 3; ++j) centers[3 * k + j] = ic.centers[k][j];
+    amps[k] = ic.amps[k];
+  }
+  return ok();
+}
+
+int lpm_ic_eval(const double* xyz, int64_t n, double* zeta, double sbr, int rh_n, double rh_w, double rh_K,
+                int nvort, const double* centers, const double* amps, double sigma) {
+  if (n < 0 || nvort < 0 || (nvort > 0 && (!centers || !amps)))
+    return fail(LPM_ERR_ARG, "lpm_ic_eval: bad arguments");
+  CompositeVorticity ic;
+  ic.sbr = sbr;
+  ic.rh_n = rh_n;
+  ic.rh_w = rh_w;
+  ic.rh_K = rh_K;
+  ic.sigma = sigma;
+  for (int k = 0; k < nvort; ++k) {
+    ic.centers.push_back(Vec3{{centers[3 * k], centers[3 * k + 1], centers[3 * k + 2]}});
+    ic.amps.push_back(amps[k]);
+  }
+  // per-particle evaluation, as BVESphere::init_vorticity (src/LpmBVESphere.cpp:126-149)
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) zeta[i] = ic.eval(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  return ok();
+}
+
+int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, double* mean) {
+  double num = 0.0, den = 0.0;
+  for (int64_t i = 0; i < nf; ++i) {
+    if (fmask && fmask[i]) continue;
+    num += fzeta[i] * farea[i];
+    den += farea[i];
+  }
+  if (den == 0.0) return fail(LPM_ERR_ARG, "lpm_area_mean: zero total area");
+  *mean = num / den;
+  return ok();
+}
+
+}  // extern "C"

@@ -1,0 +1,270 @@
+// Host mesh build: icosahedral seed and 4-way tree refinement.
+// Restates src/LpmMeshSeed.cpp:20-232,296-320, src/LpmPolyMesh2d.cpp:10-44,
+// src/LpmFaces.cpp:9-63,115-248 and src/LpmEdges.cpp:12-85 (see
+// lpm_mesh.hpp).  Compiled with -ffp-contract=off.
+#include "lpm_mesh.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace lpm {
+
+namespace {
+
+constexpr int kNone = -1;
+
+const SeedTables kIcosSeed = {
+    {
+        {{0.0, 0.0, 1.0}},
+        {{0.7236067977499789, 0.5257311121191336, 0.4472135954999579}},
+        {{-0.276393202250021, 0.8506508083520399, 0.4472135954999579}},
+        {{-0.8944271909999159, 0.0, 0.4472135954999579}},
+        {{-0.276393202250021, -0.8506508083520399, 0.4472135954999579}},
+        {{0.7236067977499789, -0.5257311121191336, 0.4472135954999579}},
+        {{0.8944271909999159, 0.0, -0.4472135954999579}},
+        {{0.276393202250021, 0.8506508083520399, -0.4472135954999579}},
+        {{-0.7236067977499789, 0.5257311121191336, -0.4472135954999579}},
+        {{-0.7236067977499789, -0.5257311121191336, -0.4472135954999579}},
+        {{0.276393202250021, -0.8506508083520399, -0.4472135954999579}},
+        {{0.0, 0.0, -1.0}},
+        {{0.18759247408507992, 0.5773502691896258, 0.7946544722917662}},
+        {{-0.491123473188423, 0.35682208977308993, 0.7946544722917661}},
+        {{-0.491123473188423, -0.35682208977308993, 0.7946544722917661}},
+        {{0.18759247408507992, -0.5773502691896258, 0.7946544722917662}},
+        {{0.6070619982066863, 0.0, 0.7946544722917662}},
+        {{0.7946544722917661, 0.5773502691896257, -0.1875924740850799}},
+        {{0.30353099910334314, 0.9341723589627157, 0.18759247408507992}},
+        {{-0.30353099910334314, 0.9341723589627157, -0.18759247408507992}},
+        {{-0.7946544722917661, 0.5773502691896257, 0.1875924740850799}},
+        {{-0.982246946376846, 0.0, -0.1875924740850799}},
+        {{-0.7946544722917661, -0.5773502691896257, 0.1875924740850799}},
+        {{-0.30353099910334314, -0.9341723589627157, -0.18759247408507992}},
+        {{0.30353099910334314, -0.9341723589627157, 0.18759247408507992}},
+        {{0.7946544722917661, -0.5773502691896257, -0.1875924740850799}},
+        {{0.982246946376846, 0.0, 0.1875924740850799}},
+        {{0.491123473188423, 0.35682208977308993, -0.7946544722917661}},
+        {{-0.18759247408507992, 0.5773502691896258, -0.7946544722917662}},
+        {{-0.6070619982066863, 0.0, -0.7946544722917662}},
+        {{-0.18759247408507992, -0.5773502691896258, -0.7946544722917662}},
+        {{0.491123473188423, -0.35682208977308993, -0.7946544722917661}},
+    },
+    {{0, 1, 0, 4}, {1, 2, 0, 6}, {2, 0, 0, 1}, {2, 3, 1, 8}, {0, 3, 2, 1}, {3, 4, 2, 10}, {4, 0, 2, 3}, {4, 5, 3, 12}, {5, 0, 3, 4}, {5, 1, 4, 14}, {1, 6, 5, 14}, {6, 7, 5, 15}, {7, 1, 5, 6}, {7, 2, 6, 7}, {7, 8, 7, 16}, {8, 2, 7, 8}, {8, 3, 8, 9}, {8, 9, 9, 17}, {3, 9, 10, 9}, {9, 4, 10, 11}, {9, 10, 11, 18}, {10, 4, 11, 12}, {10, 5, 12, 13}, {10, 6, 13, 19}, {6, 5, 13, 14}, {11, 6, 19, 15}, {11, 7, 15, 16}, {8, 11, 17, 16}, {11, 9, 17, 18}, {10, 11, 19, 18}},
+    {{0, 1, 2}, {0, 2, 3}, {0, 3, 4}, {0, 4, 5}, {0, 5, 1}, {1, 6, 7}, {7, 2, 1}, {2, 7, 8}, {8, 3, 2}, {3, 8, 9}, {9, 4, 3}, {4, 9, 10}, {10, 5, 4}, {5, 10, 6}, {6, 1, 5}, {11, 7, 6}, {11, 8, 7}, {11, 9, 8}, {11, 10, 9}, {11, 6, 10}},
+    {{0, 1, 2}, {2, 3, 4}, {4, 5, 6}, {6, 7, 8}, {8, 9, 0}, {10, 11, 12}, {13, 1, 12}, {13, 14, 15}, {16, 3, 15}, {16, 17, 18}, {19, 5, 18}, {19, 20, 21}, {22, 7, 21}, {22, 23, 24}, {10, 9, 24}, {26, 11, 25}, {27, 14, 26}, {28, 17, 27}, {29, 20, 28}, {25, 23, 29}},
+};
+
+// Growing edge and face tables of the tree.  Edges store origin,
+// destination, left and right face, and the two children of a divided
+// edge (child 0 starts at the origin).
+struct Builder {
+  TreeMesh& m;
+  std::vector<int> e_orig, e_dest, e_left, e_right, e_kid0, e_kid1;
+  std::vector<int> f_edges;  // [nfaces][3]
+  std::vector<int> f_kid0;   // first child of a divided face, else -1
+
+  explicit Builder(TreeMesh& mesh) : m(mesh) {}
+
+  int add_edge(int o, int d, int l, int r) {
+    e_orig.push_back(o);
+    e_dest.push_back(d);
+    e_left.push_back(l);
+    e_right.push_back(r);
+    e_kid0.push_back(kNone);
+    e_kid1.push_back(kNone);
+    return (int)e_orig.size() - 1;
+  }
+
+  int add_vertex(const Vec3& phys, const Vec3& lag) {
+    m.vert_phys.push_back(phys);
+    m.vert_lag.push_back(lag);
+    return (int)m.vert_phys.size() - 1;
+  }
+
+  // Split edge e at its projected midpoint; children inherit both faces.
+  void split_edge(int e) {
+    const int o = e_orig[e], d = e_dest[e], l = e_left[e], r = e_right[e];
+    const int mid = add_vertex(sphere::midpoint(m.vert_phys[o], m.vert_phys[d]),
+                               sphere::midpoint(m.vert_lag[o], m.vert_lag[d]));
+    const int c0 = add_edge(o, mid, l, r);
+    const int c1 = add_edge(mid, d, l, r);
+    e_kid0[e] = c0;
+    e_kid1[e] = c1;
+  }
+
+  void add_face(const int v[3], const int e[3], int parent, const Vec3& ctr, const Vec3& lagctr, double area) {
+    for (int k = 0; k < 3; ++k) {
+      m.face_verts.push_back(v[k]);
+      f_edges.push_back(e[k]);
+    }
+    f_kid0.push_back(kNone);
+    m.face_parent.push_back(parent);
+    m.face_phys.push_back(ctr);
+    m.face_lag.push_back(lagctr);
+    m.face_area.push_back(area);
+    m.face_mask.push_back(0);
+  }
+
+  // Split face f into corner children 0..2 (child k keeps corner k) and the
+  // centre child 3.
+  void split_face(int f) {
+    const int first = (int)m.face_phys.size();
+    int cv[4][3], ce[4][3];
+    for (auto& row : cv) row[0] = row[1] = row[2] = kNone;
+    for (auto& row : ce) row[0] = row[1] = row[2] = kNone;
+    for (int k = 0; k < 3; ++k) cv[k][k] = m.face_verts[3 * f + k];
+    for (int k = 0; k < 3; ++k) {
+      const int pe = f_edges[3 * f + k];
+      if (!(e_kid0[pe] > 0)) split_edge(pe);
+      const int h0 = e_kid0[pe], h1 = e_kid1[pe];
+      const int next = (k + 1) % 3;
+      if (e_left[pe] == f) {  // edge runs counter-clockwise around f
+        ce[k][k] = h0;
+        e_left[h0] = first + k;
+        ce[next][k] = h1;
+        e_left[h1] = first + next;
+      } else {
+        ce[k][k] = h1;
+        e_right[h1] = first + k;
+        ce[next][k] = h0;
+        e_right[h0] = first + next;
+      }
+      const int mid = e_dest[h0];
+      // midpoint of parent edge k is shared by corner children k and k+1
+      // and by the centre child (opposite corner k+2)
+      cv[k][next] = mid;
+      cv[next][k] = mid;
+      cv[3][(k + 2) % 3] = mid;
+    }
+    const int e0 = (int)e_orig.size();
+    ce[3][0] = e0;
+    ce[3][1] = e0 + 1;
+    ce[3][2] = e0 + 2;
+    ce[0][1] = e0 + 1;
+    ce[1][2] = e0 + 2;
+    ce[2][0] = e0;
+    add_edge(cv[2][1], cv[2][0], first + 3, first + 2);
+    add_edge(cv[0][2], cv[0][1], first + 3, first + 0);
+    add_edge(cv[1][0], cv[1][2], first + 3, first + 1);
+    for (int c = 0; c < 4; ++c) {
+      Vec3 phys[3], lag[3];
+      for (int k = 0; k < 3; ++k) {
+        phys[k] = m.vert_phys[cv[c][k]];
+        lag[k] = m.vert_lag[cv[c][k]];
+      }
+      const Vec3 ctr = sphere::barycenter(phys, 3);
+      const Vec3 lagctr = sphere::barycenter(lag, 3);
+      add_face(cv[c], ce[c], f, ctr, lagctr, sphere::polygon_area(ctr, phys, 3));
+    }
+    f_kid0[f] = first;
+    m.face_area[f] = 0.0;
+    m.face_mask[f] = 1;
+  }
+};
+
+}  // namespace
+
+const SeedTables& builtin_icos_seed() { return kIcosSeed; }
+
+bool read_icos_seed(const std::string& path, SeedTables& out, std::string& err) {
+  std::ifstream in(path);
+  if (!in) {
+    err = "cannot open seed file " + path;
+    return false;
+  }
+  const int ncrd = IcosTriSphereSeed::nverts + IcosTriSphereSeed::nfaces;
+  int edge_hdr = -1, fv_hdr = -1, fe_hdr = -1, n_crd = 0, n_e = 0, n_fv = 0, n_fe = 0, line_no = 0;
+  std::string line;
+  while (std::getline(in, line)) {
+    ++line_no;
+    if (line.find("edgeO") != std::string::npos) edge_hdr = line_no;
+    if (line.find("faceverts") != std::string::npos) fv_hdr = line_no;
+    if (line.find("faceedges") != std::string::npos) fe_hdr = line_no;
+    std::istringstream ss(line);
+    bool ok = true;
+    if (line_no > 1 && line_no < ncrd + 2) {
+      Vec3& p = out.crds[n_crd++];
+      ok = bool(ss >> p[0] >> p[1] >> p[2]);
+    } else if (edge_hdr > 0 && line_no > edge_hdr && line_no <= edge_hdr + IcosTriSphereSeed::nedges) {
+      int* e = out.edges[n_e++];
+      ok = bool(ss >> e[0] >> e[1] >> e[2] >> e[3]);
+    } else if (fv_hdr > 0 && line_no > fv_hdr && line_no <= fv_hdr + IcosTriSphereSeed::nfaces) {
+      int* v = out.faceverts[n_fv++];
+      ok = bool(ss >> v[0] >> v[1] >> v[2]);
+    } else if (fe_hdr > 0 && line_no > fe_hdr && line_no <= fe_hdr + IcosTriSphereSeed::nfaces) {
+      int* e = out.faceedges[n_fe++];
+      ok = bool(ss >> e[0] >> e[1] >> e[2]);
+    }
+    if (!ok) {
+      err = "malformed line " + std::to_string(line_no) + " in " + path;
+      return false;
+    }
+  }
+  if (n_crd != ncrd || n_e != IcosTriSphereSeed::nedges || n_fv != IcosTriSphereSeed::nfaces ||
+      n_fe != IcosTriSphereSeed::nfaces) {
+    err = "incomplete seed file " + path;
+    return false;
+  }
+  return true;
+}
+
+void icos_max_allocations(int lev, int64_t& nverts, int64_t& nedges, int64_t& nfaces) {
+  nverts = IcosTriSphereSeed::nVerticesAtTreeLevel(lev);
+  nedges = 0;
+  nfaces = 0;
+  for (int i = 0; i <= lev; ++i) {
+    nfaces += IcosTriSphereSeed::nFacesAtTreeLevel(i);
+    nedges += IcosTriSphereSeed::nEdgesAtTreeLevel(IcosTriSphereSeed::nVerticesAtTreeLevel(i),
+                                                   IcosTriSphereSeed::nFacesAtTreeLevel(i));
+  }
+}
+
+int64_t TreeMesh::nleaves() const {
+  int64_t n = 0;
+  for (uint8_t k : face_mask) n += (k == 0);
+  return n;
+}
+
+TreeMesh build_icos_tree(int lev, const SeedTables& seed) {
+  TreeMesh m;
+  m.level = lev;
+  int64_t nv, ne, nf;
+  icos_max_allocations(lev, nv, ne, nf);
+  m.vert_phys.reserve(nv);
+  m.vert_lag.reserve(nv);
+  m.face_phys.reserve(nf);
+  m.face_lag.reserve(nf);
+  m.face_area.reserve(nf);
+  m.face_mask.reserve(nf);
+  m.face_verts.reserve(3 * nf);
+  m.face_parent.reserve(nf);
+  Builder b(m);
+  b.e_orig.reserve(ne);
+  b.e_dest.reserve(ne);
+  b.e_left.reserve(ne);
+  b.e_right.reserve(ne);
+  b.e_kid0.reserve(ne);
+  b.e_kid1.reserve(ne);
+  b.f_edges.reserve(3 * nf);
+  b.f_kid0.reserve(nf);
+  for (int i = 0; i < IcosTriSphereSeed::nverts; ++i) b.add_vertex(seed.crds[i], seed.crds[i]);
+  for (int i = 0; i < IcosTriSphereSeed::nedges; ++i)
+    b.add_edge(seed.edges[i][0], seed.edges[i][1], seed.edges[i][2], seed.edges[i][3]);
+  for (int i = 0; i < IcosTriSphereSeed::nfaces; ++i) {
+    // seed faces keep the seed's centre coordinates; the area uses them too
+    const Vec3& ctr = seed.crds[IcosTriSphereSeed::nverts + i];
+    Vec3 corners[3];
+    for (int k = 0; k < 3; ++k) corners[k] = seed.crds[seed.faceverts[i][k]];
+    b.add_face(seed.faceverts[i], seed.faceedges[i], kNone, ctr, ctr, sphere::polygon_area(ctr, corners, 3));
+  }
+  for (int l = 0; l < lev; ++l) {
+    const int stop = (int)m.face_phys.size();
+    for (int f = 0; f < stop; ++f)
+      if (!(b.f_kid0[f] > 0)) b.split_face(f);
+  }
+  m.n_edges = (int64_t)b.e_orig.size();
+  return m;
+}
+
+}  // namespace lpm

@@ -1,0 +1,73 @@
+// Icosahedral triangle tree mesh on the unit sphere, built on the host.
+//
+// Mirrors MeshSeed<IcosTriSphereSeed> (src/LpmMeshSeed.hpp:132-147,
+// :181-224) and PolyMesh2d<Seed>::treeInit (src/LpmPolyMesh2d.cpp:30-44):
+// every leaf face is split 4 ways, level by level, with edge midpoints
+// projected to the sphere.  Vertex, face and edge numbering, positions and
+// areas are identical to the reference's, so particle i here is particle i
+// there.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lpm_geometry.hpp"
+
+namespace lpm {
+
+struct IcosTriSphereSeed {
+  static constexpr int nverts = 12;
+  static constexpr int nfaces = 20;
+  static constexpr int nedges = 30;
+  static constexpr int nfaceverts = 3;
+  static const char* filename() { return "icosTriSphereSeed.dat"; }
+  static const char* idString() { return "IcosTriSphereSeed"; }
+  // Counts at tree level `lev` (src/LpmMeshSeed.cpp:381-391).
+  static int64_t nVerticesAtTreeLevel(int lev) { return 2 + 10 * (int64_t(1) << (2 * lev)); }
+  static int64_t nFacesAtTreeLevel(int lev) { return 20 * (int64_t(1) << (2 * lev)); }
+  static int64_t nEdgesAtTreeLevel(int64_t nv, int64_t nf) { return nv + nf - 2; }
+};
+
+// Seed data: 12 vertices then 20 face centres, 30 edges (origin, destination,
+// left face, right face), 20 faces (vertices, edges).
+struct SeedTables {
+  Vec3 crds[32];
+  int edges[30][4];
+  int faceverts[20][3];
+  int faceedges[20][3];
+};
+
+// The icosahedral seed shipped with the library (the values of the
+// reference's mesh_seeds/icosTriSphereSeed.dat, needed bit for bit for
+// identical meshes).
+const SeedTables& builtin_icos_seed();
+
+// Parse a seed file in the reference's format (src/LpmMeshSeed.cpp:20-232).
+// Returns false and sets `err` on failure.
+bool read_icos_seed(const std::string& path, SeedTables& out, std::string& err);
+
+// Allocation sizes for a tree of depth `lev` (MeshSeed::setMaxAllocations,
+// src/LpmMeshSeed.cpp:296-305): vertices, edges, faces (all tree levels).
+void icos_max_allocations(int lev, int64_t& nverts, int64_t& nedges, int64_t& nfaces);
+
+// The built mesh.  Faces include every tree level (parents first, leaves
+// last); divided faces have area 0 and mask 1.
+struct TreeMesh {
+  int level = 0;
+  std::vector<Vec3> vert_phys, vert_lag;
+  std::vector<Vec3> face_phys, face_lag;
+  std::vector<double> face_area;
+  std::vector<uint8_t> face_mask;
+  std::vector<int> face_verts;   // [nfaces][3]
+  std::vector<int> face_parent;  // -1 for seed faces
+  int64_t n_edges = 0;
+  int64_t nverts() const { return (int64_t)vert_phys.size(); }
+  int64_t nfaces() const { return (int64_t)face_phys.size(); }
+  int64_t nleaves() const;
+};
+
+// PolyMesh2d::treeInit: seed, then `lev` rounds of 4-way refinement.
+TreeMesh build_icos_tree(int lev, const SeedTables& seed);
+
+}  // namespace lpm

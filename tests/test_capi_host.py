@@ -1,0 +1,88 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol
+include/lpm_bve.h declares, and its host pieces (mesh build, initial
+conditions) match the oracle bit for bit."""
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+import lpmkokkos_amd as L
+from lpmkokkos_amd import _lib
+
+
+def declared_symbols():
+    text = open(_lib.HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s+(lpm_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    lib = _lib.lib()
+    for name in syms:
+        assert hasattr(lib, name), name
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+
+
+def test_error_reporting():
+    lib = _lib.lib()
+    nv = C.c_int64()
+    rc = lib.lpm_icos_alloc(-1, C.byref(nv), None, None)
+    assert rc == 1 and b"level" in lib.lpm_last_error()
+    assert lib.lpm_icos_alloc(3, C.byref(nv), None, None) == 0 and nv.value == 642
+    assert lib.lpm_last_error() == b""
+    with pytest.raises(L.LpmError):
+        L.TreeMesh(13)
+    rc = lib.lpm_icos_tree_init(2, b"/nonexistent/seed.dat", None, None, None, None, None, None, None, None, None)
+    assert rc == 6
+
+
+def test_seed_fixture_matches_builtin(oracle_mod):
+    from oracle.oracle import SEED_FIXTURE
+    for level in (0, 3):
+        a = L.TreeMesh(level)
+        b = L.TreeMesh(level, L.MeshSeed(path=SEED_FIXTURE))
+        for k in ("vx", "fx", "farea", "fmask", "fverts", "fparent"):
+            assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3, 4, 5, 6])
+def test_mesh_bitwise_equals_oracle(omesh, level):
+    m = L.TreeMesh(level)
+    o = omesh(level)
+    assert (m.nverts, m.nedges, m.nfaces, m.nleaves) == (o.nv, o.ne, o.nf, o.nleaves)
+    for a, b in ((m.vx, o.vx), (m.vlag, o.vlag), (m.fx, o.fx), (m.flag, o.flag), (m.farea, o.farea),
+                 (m.fmask, o.fmask), (m.fverts, o.fverts), (m.fparent, o.fparent)):
+        assert np.array_equal(a, b)
+
+
+def test_ic_bitwise_equals_oracle(oracle_mod, omesh):
+    O = oracle_mod
+    m = omesh(4)
+    c, a = L.gauss_params(7, 32, 1.0)
+    co, ao = O.gauss_params(7, 32, 1.0)
+    assert np.array_equal(c, co) and np.array_equal(a, ao)
+    assert np.allclose(np.linalg.norm(c, axis=1), 1.0, atol=1e-15)
+    ic = L.VorticityInitialCondition(sbr=0.5, rh_n=4, rh_w=0.1, rh_K=0.1, centers=c, amps=a, sigma=0.1)
+    for x in (m.vx, m.fx):
+        z = ic.eval_many(x)
+        zo = O.ic_eval(x, sbr=0.5, rh_n=4, rh_w=0.1, rh_K=0.1, centers=co, amps=ao, sigma=0.1)
+        assert np.array_equal(z, zo)
+    sbr = L.SolidBodyRotation()
+    assert sbr.eval(0.0, 0.6, 0.8) == 2 * (2 * L.bve.PI) * 0.8
+
+
+def test_area_mean(omesh):
+    m = omesh(3)
+    z = np.arange(m.nf, dtype=np.float64)
+    mu = L.area_mean(z, m.farea, m.fmask)
+    lv = m.fmask == 0
+    assert abs(mu - (z[lv] * m.farea[lv]).sum() / m.farea[lv].sum()) < 1e-9
+
+
+def test_rh_phase_speed():
+    ic = L.RossbyHaurwitzWave(4, 0.1, 0.1)
+    # nu = (n(n+3) w - 2 Omega) / ((n+1)(n+2)) = (2.8 - 4 pi) / 30
+    assert abs(ic.phase_speed(2 * np.pi) - (2.8 - 4 * np.pi) / 30) < 1e-15

@@ -1,0 +1,243 @@
+"""Parity of the HIP path (through the C ABI of liblpm_bve.so) with the CPU
+oracle.  Tolerances (normwise, src/LpmErrorNorms.cpp:33-44):
+  FP64 reference scheme: velocity 1e-11, positions 1e-9 after 100 steps;
+  single evaluations 1e-13 (summation order differs from the oracle's);
+  FP32C vs FP64: velocity 1e-6, stream function 1e-6.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import lpmkokkos_amd as L
+from helpers import LeafMesh, gauss_field, normwise, rh_field
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * math.pi
+
+
+def make_ctx(lm, vz, fz):
+    c = L.Context(0)
+    c.set_state(lm.vx, vz, lm.fx, fz, lm.farea, lm.fmask)
+    return c
+
+
+def test_functor_entry_points(omesh):
+    """BVE{Vertex,Face}{Velocity,StreamFn} on caller-owned device arrays over all
+    tree faces (masked loop), against the oracle at vertices and active faces."""
+    import torch
+    m = omesh(4)
+    fz = rh_field(m.fx)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    vx, fx, z, a, msk = t(m.vx), t(m.fx), t(fz), t(m.farea), t(m.fmask)
+    uv = torch.zeros_like(vx)
+    uf = torch.zeros_like(fx)
+    pv = torch.zeros(m.nv, dtype=torch.float64, device=dev)
+    pf = torch.zeros(m.nf, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    P = L._lib.ptr
+    L._lib.call("lpm_bve_vertex_velocity", P(uv), P(vx), m.nv, P(fx), P(z), P(a), P(msk), m.nf, st)
+    L._lib.call("lpm_bve_face_velocity", P(uf), P(fx), P(z), P(a), P(msk), m.nf, st)
+    L._lib.call("lpm_bve_vertex_streamfn", P(pv), P(vx), m.nv, P(fx), P(z), P(a), P(msk), m.nf, st)
+    L._lib.call("lpm_bve_face_streamfn", P(pf), P(fx), P(z), P(a), P(msk), m.nf, st)
+    torch.cuda.synchronize()
+    lv = m.fmask == 0
+    ov = O.velocity(m.vx, m.fx, fz, m.farea, m.fmask, False)
+    of = O.velocity(m.fx, m.fx, fz, m.farea, m.fmask, True)
+    opv = O.streamfn(m.vx, m.fx, fz, m.farea, m.fmask, False)
+    opf = O.streamfn(m.fx, m.fx, fz, m.farea, m.fmask, True)
+    assert normwise(uv.cpu().numpy(), ov) < 1e-13
+    assert normwise(uf.cpu().numpy()[lv], of[lv]) < 1e-13
+    assert normwise(pv.cpu().numpy(), opv) < 1e-13
+    assert normwise(pf.cpu().numpy()[lv], opf[lv]) < 1e-13
+
+
+@pytest.mark.parametrize("level", [0, 2, 4])
+def test_solve_fp64_ref(omesh, level):
+    m = omesh(level)
+    lm = LeafMesh(m)
+    vz, fz = rh_field(lm.vx), rh_field(lm.fx)
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_REF, want_psi=True)
+    s = c.get_state()
+    ov = O.velocity(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False)
+    of = O.velocity(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True)
+    assert normwise(s["vu"], ov) < 1e-13
+    assert normwise(s["fu"], of) < 1e-13
+    assert normwise(s["vpsi"], O.streamfn(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False)) < 1e-13
+    assert normwise(s["fpsi"], O.streamfn(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True)) < 1e-13
+    c.close()
+
+
+def test_tree_faces_drop_in(omesh):
+    """Passing every tree face (the reference's arrays): active faces and
+    vertices match the oracle; divided faces keep position, zero velocity."""
+    m = omesh(3)
+    vz, fz = rh_field(m.vx), rh_field(m.fx)
+    c = L.Context(0)
+    c.set_state(m.vx, vz, m.fx, fz, m.farea, m.fmask)
+    c.solve(L.PREC_FP64, L.SCHEME_REF)
+    s = c.get_state()
+    lv = m.fmask == 0
+    of = O.velocity(m.fx, m.fx, fz, m.farea, m.fmask, True)
+    assert normwise(s["fu"][lv], of[lv]) < 1e-13
+    assert np.all(s["fu"][~lv] == 0) and np.array_equal(s["fx"][~lv], m.fx[~lv])
+    assert c.sizes() == (m.nv, m.nf, m.nleaves, m.nv + m.nleaves)
+    c.close()
+
+
+def test_c0_rh_wave_100_steps_ref(omesh):
+    """Config 0: level 4, RH n=4 w=K=0.1, Omega=2 pi, dt=0.01, 100 steps of the
+    reference scheme in FP64: velocity <= 1e-11, positions <= 1e-9 (normwise)."""
+    m = omesh(4)
+    lm = LeafMesh(m)
+    vz, fz = rh_field(lm.vx), rh_field(lm.fx)
+    o = O.State(lm, vz, fz)
+    o.init_velocity(0)
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_REF)
+    c.advance(0.01, TWO_PI, L.SCHEME_REF, L.PREC_FP64, nsteps=100)
+    for _ in range(100):
+        o.step(0.01, TWO_PI, 0)
+    s = c.get_state()
+    ev = max(normwise(s["vu"], o.vvel), normwise(s["fu"], o.fvel))
+    ex = max(normwise(s["vx"], o.vx), normwise(s["fx"], o.fx))
+    ez = max(normwise(s["vzeta"], o.vzeta), normwise(s["fzeta"], o.fzeta))
+    print(f"C0 100 steps: velocity {ev:.3e} positions {ex:.3e} vorticity {ez:.3e}")
+    assert ev <= 1e-11 and ex <= 1e-9 and ez <= 1e-9
+    c.close()
+
+
+def test_lagr_scheme_fp64(omesh):
+    m = omesh(3)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=3, nvort=8, sigma=0.15, rh=(4, 0.1, 0.1))
+    fz = fz[m.fmask == 0]
+    o = O.State(lm, vz, fz)
+    o.set_lagrangian(TWO_PI)
+    o.init_velocity(1)
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR)
+    c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP64, nsteps=20)
+    for _ in range(20):
+        o.step(0.005, TWO_PI, 1)
+    s = c.get_state()
+    assert max(normwise(s["vu"], o.vvel), normwise(s["fu"], o.fvel)) < 1e-11
+    assert max(normwise(s["vx"], o.vx), normwise(s["fx"], o.fx)) < 1e-11
+    assert np.abs(np.linalg.norm(s["fx"], axis=1) - 1).max() < 1e-15
+    assert np.abs(s["fzeta"] + 2 * TWO_PI * s["fx"][:, 2] - (fz + 2 * TWO_PI * lm.fx[:, 2])).max() < 1e-12
+    c.close()
+
+
+@pytest.mark.parametrize("level", [5, 6])
+def test_fp32c_vs_fp64(omesh, level):
+    """FP32C pair sums against FP64 on the Gaussian-vortex field plus RH wave:
+    velocity and stream function within 1e-6 normwise."""
+    m = omesh(level)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=1, nvort=32, sigma=0.1, rh=(4, 0.1, 0.1))
+    fz = fz[m.fmask == 0]
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
+    a = c.get_state()
+    c.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
+    b = c.get_state()
+    eu = max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"]))
+    ep = max(normwise(b["vpsi"], a["vpsi"]), normwise(b["fpsi"], a["fpsi"]))
+    print(f"level {level}: FP32C velocity {eu:.3e} psi {ep:.3e}")
+    assert eu < 1e-6 and ep < 1e-6
+    # the FP64 chord form agrees with the oracle's chord form
+    ov = O.velocity_chord(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False)
+    assert normwise(a["vu"], ov) < 1e-13
+    c.close()
+
+
+def test_graph_replay_matches_eager(omesh):
+    """Steps replayed from the captured hipGraph give the same bits as eager
+    launches, replay after replay with eager work in between."""
+    m = omesh(4)
+    lm = LeafMesh(m)
+    vz, fz = rh_field(lm.vx), rh_field(lm.fx)
+    a = make_ctx(lm, vz, fz)
+    b = make_ctx(lm, vz, fz)
+    for c in (a, b):
+        c.solve(L.PREC_FP64, L.SCHEME_LAGR)
+    for rep in range(3):
+        a.advance(0.01, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=2, use_graph=True)
+        b.advance(0.01, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=2, use_graph=False)
+        sa, sb = a.get_state(), b.get_state()
+        a.circulation()
+        for k in ("vx", "fx", "vu", "fu", "fzeta"):
+            assert np.array_equal(sa[k], sb[k]), (rep, k)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("R,splits", [(1, 1), (2, 3), (4, 0), (4, 7)])
+def test_launch_shapes_agree(omesh, R, splits):
+    m = omesh(4)
+    lm = LeafMesh(m)
+    vz, fz = rh_field(lm.vx), rh_field(lm.fx)
+    c = make_ctx(lm, vz, fz)
+    ov = O.velocity(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False)
+    of = O.velocity(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True)
+    c.set_launch(R, splits)
+    for prec, scheme, tol in ((L.PREC_FP64, L.SCHEME_REF, 1e-13), (L.PREC_FP32C, L.SCHEME_LAGR, 1e-6)):
+        c.solve(prec, scheme)
+        s = c.get_state()
+        assert normwise(s["vu"], ov) < tol and normwise(s["fu"], of) < tol, (prec, R, splits)
+    c.close()
+
+
+def test_c1_level7_subset(omesh):
+    """Config 1: level 7, 32 Gaussian vortices (sigma 0.1, seed 1, mean removed):
+    FP64 reference-scheme velocity against the oracle on every 997th target."""
+    from oracle.oracle import Mesh
+    m = Mesh(7)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=1, nvort=32, sigma=0.1)
+    fz = fz[m.fmask == 0]
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_REF)
+    s = c.get_state()
+    stride = 997
+    ov = O.velocity(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False, stride=stride)[::stride]
+    of = O.velocity(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True, stride=stride)[::stride]
+    assert normwise(s["vu"][::stride], ov) < 1e-13
+    assert normwise(s["fu"][::stride], of) < 1e-13
+    c.close()
+
+
+def test_solid_body_rotation_gpu(omesh):
+    """tests/LpmBVETest.cpp: zeta = 2 Omega z moves particles rigidly."""
+    m = omesh(4)
+    lm = LeafMesh(m)
+    vz = O.ic_eval(lm.vx, sbr=1.0)
+    fz = O.ic_eval(lm.fx, sbr=1.0)
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_REF)
+    dt, n = 0.01, 20
+    c.advance(dt, TWO_PI, L.SCHEME_REF, L.PREC_FP64, nsteps=n)
+    s = c.get_state()
+    t = dt * n
+    cs, sn = math.cos(TWO_PI * t), math.sin(TWO_PI * t)
+    ex = np.stack([lm.vx[:, 0] * cs - lm.vx[:, 1] * sn, lm.vx[:, 1] * cs + lm.vx[:, 0] * sn, lm.vx[:, 2]], 1)
+    assert np.abs(s["vx"] - ex).max() < 5e-3
+    assert np.abs(np.einsum("ij,ij->i", s["vu"], s["vx"])).max() < 1e-3
+    c.close()
+
+
+def test_circulation(omesh):
+    m = omesh(5)
+    lm = LeafMesh(m)
+    vz, fz = rh_field(lm.vx), rh_field(lm.fx)
+    c = make_ctx(lm, vz, fz)
+    g0 = c.circulation()
+    assert abs(g0 - float(np.sum(fz * lm.farea))) < 1e-12
+    # zeta = absv - 2 Omega z: the sum moves only with the quadrature of z dA
+    c.advance(0.01, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=5)
+    assert abs(c.circulation() - g0) < 1e-6 * float(np.sum(np.abs(fz) * lm.farea))
+    c.close()

@@ -1,0 +1,288 @@
+"""Pins the CPU oracle against the reference's own known answers and against
+what the mathematics fixes.  No GPU."""
+import math
+
+import numpy as np
+import pytest
+
+PI = 3.1415926535897932384626433832795027975
+
+# scripts/poissonOutput.py:6-30: icosahedral rows, identical for the serial and
+# GPU builds: (mesh size in degrees, psi L-inf at vertices, at faces)
+POISSON_ICOS = [
+    (45.4164, 2.42561, 1.49743),
+    (22.7082, 0.175552, 0.446258),
+    (11.3541, 0.0327947, 0.18008),
+    (5.67705, 0.0078577, 0.057519),
+    (2.83852, 0.00199034, 0.0170644),
+    (1.41926, 0.000499979, 0.00492556),
+]
+
+
+def sig6(a, b):
+    """Equal when printed with 6 significant digits (std::cout default)."""
+    return float(f"{a:.6g}") == float(f"{b:.6g}")
+
+
+@pytest.mark.parametrize("level", range(6))
+def test_poisson_table(oracle_mod, omesh, level):
+    """Y_5^4 vorticity: stream-function sum vs psi = Y/30, as the reference's
+    SpherePoisson test (tests/LpmSpherePoisson.cpp:13-51, src/LpmSpherePoisson.hpp:428-475).
+    Pins every position, area and mask of the mesh and the psi pair sum."""
+    O = oracle_mod
+    m = omesh(level)
+    fv, _ = O.sphharm54(m.vx)
+    ff, _ = O.sphharm54(m.fx)
+    pv = O.streamfn(m.vx, m.fx, ff, m.farea, m.fmask, False)
+    pf = O.streamfn(m.fx, m.fx, ff, m.farea, m.fmask, True)
+    ev = np.abs(pv - fv / 30).max()
+    ef = np.where(m.fmask == 1, 0.0, np.abs(pf - ff / 30)).max()
+    h = math.degrees(math.sqrt(4 * PI / m.nleaves))
+    want = POISSON_ICOS[level]
+    assert sig6(h, want[0])
+    assert sig6(ev, want[1]), (ev, want[1])
+    assert sig6(ef, want[2]), (ef, want[2])
+
+
+def test_poisson_velocity_converges(oracle_mod, omesh):
+    """Velocity of the Y_5^4 field against RH54Velocity (src/LpmRossbyWaves.hpp:39-53):
+    the point-vortex quadrature of the singular kernel converges at vertices
+    (about h^0.7 here); a wrong sign or a dropped term would leave O(1) errors."""
+    O = oracle_mod
+    errs = []
+    for level in (3, 4, 5):
+        m = omesh(level)
+        ff, _ = O.sphharm54(m.fx)
+        _, uex = O.sphharm54(m.vx)
+        u = O.velocity(m.vx, m.fx, ff, m.farea, m.fmask, False)
+        errs.append(np.abs(u - uex).max() / np.abs(uex).max())
+    assert errs[0] < 0.02 and errs[0] / errs[1] > 1.4 and errs[1] / errs[2] > 1.4, errs
+
+
+def test_seed_values(oracle_mod):
+    """Seed file: 12 unit vertices; face centres are normalised vertex means
+    (mesh_seeds/icosTriSphereSeed.dat:2-33)."""
+    crds, edges, fv, fe = oracle_mod.read_seed()
+    assert np.allclose(np.linalg.norm(crds, axis=1), 1.0, atol=2e-16 * 4)
+    for f in range(20):
+        c = crds[fv[f]].mean(axis=0)
+        c /= np.linalg.norm(c)
+        assert np.allclose(c, crds[12 + f], atol=1e-15)
+    # every edge bounds the two faces it names
+    for o, d, l, r in edges:
+        assert o in fv[l] and d in fv[l] and o in fv[r] and d in fv[r]
+
+
+@pytest.mark.parametrize("level", range(7))
+def test_alloc_and_topology(oracle_mod, omesh, level):
+    """Closed-form counts (src/LpmMeshSeed.cpp:381-391), area = 4 pi,
+    Euler characteristic of the leaf mesh, parents masked with area 0."""
+    m = omesh(level)
+    nv, ne, nf = oracle_mod.icos_alloc(level)
+    assert m.nv == 2 + 10 * 4 ** level == nv
+    assert m.nf == sum(20 * 4 ** i for i in range(level + 1)) == nf
+    assert m.ne == ne
+    assert m.nleaves == 20 * 4 ** level
+    leaf_edges = 3 * m.nleaves // 2
+    assert m.nv - leaf_edges + m.nleaves == 2
+    assert abs(m.farea.sum() - 4 * PI) < 1e-12
+    assert np.all(m.farea[m.fmask == 1] == 0.0)
+    assert np.all(m.fmask[-m.nleaves:] == 0) and np.all(m.fmask[:-m.nleaves] == 1)
+    assert np.allclose(np.linalg.norm(m.vx, axis=1), 1.0, atol=1e-15)
+    assert np.allclose(np.linalg.norm(m.fx, axis=1), 1.0, atol=1e-15)
+    # Lagrangian coordinates start equal to the physical ones
+    assert np.array_equal(m.vx, m.vlag) and np.array_equal(m.fx, m.flag)
+    # children of face f are 4 consecutive faces whose parent is f
+    if level:
+        kids = np.nonzero(m.fparent >= 0)[0]
+        assert np.all(m.fparent[kids].reshape(-1, 4) == m.fparent[kids][::4, None])
+
+
+def test_geometry_known_answers(oracle_mod):
+    """tests/LpmGeometryUnitTests.cpp:101-165."""
+    O = oracle_mod.lib()
+    s = 0.57735026918962584
+    p = [np.array(v, dtype=np.float64) for v in
+         ([s, -s, s], [s, -s, -s], [s, s, -s], [s, s, s])]
+    assert abs(O.orc_tri_area(p[0], p[1], p[2]) - PI / 3.0) < 1e-14
+    mid = np.zeros(3)
+    O.orc_midpoint(mid, p[0], p[1])
+    assert abs(mid[0] - 0.5 * math.sqrt(2.0)) < 1e-14
+    c = np.zeros(3)
+    O.orc_cross(c, p[0], p[1])
+    assert abs(c[0] - 2 / 3) < 1e-14 and abs(c[1] - 2 / 3) < 1e-14 and abs(c[2]) < 1e-14
+    verts = np.ascontiguousarray(np.stack(p))
+    bc = np.zeros(3)
+    O.orc_barycenter(bc, verts, 4)
+    assert bc[0] == 1 and bc[1] == 0 and bc[2] == 0
+    assert abs(O.orc_polygon_area(bc, verts, 4) - 2 * PI / 3) < 1e-14
+    n = np.array([0.0, 0.0, 1.0])
+    assert abs(O.orc_latitude(n) - PI / 2) < 1e-15
+    # atan4 quadrants (src/LpmUtilities.hpp:14-46)
+    for ang in (0.3, 2.0, 3.5, 5.9):
+        assert abs(O.orc_longitude(np.array([math.cos(ang), math.sin(ang), 0.0])) - ang) < 1e-14
+
+
+def test_blas_known_answers(oracle_mod):
+    """tests/LpmKokkosKernelsTests.cpp:19-86: update(1,x,0.5,x1,0,x2) and scal."""
+    O = oracle_mod.lib()
+    x = np.full(300, 3.0)
+    x1 = np.full(300, 2.0)
+    z = np.full(300, 100.0)
+    O.orc_blas_update(z, x, 0.5, x1, 300)
+    assert np.all(z == 4.0)
+    r = np.zeros(100)
+    O.orc_blas_scal(r, 2.0, np.full(100, 3.0), 100)
+    assert np.all(r == 6.0)
+    # RK4 combination x += (k1+k4)/6 + (k2+k3)/3 on k = 6, 3, 3, 6 -> x + 4
+    n = 5
+    xs = np.zeros(3 * n)
+    ks = [np.full(3 * n, v) for v in (6.0, 3.0, 3.0, 6.0)]
+    v = np.zeros(n)
+    vs = [np.full(n, w) for w in (6.0, 3.0, 3.0, 6.0)]
+    O.orc_rk4_combine(xs, *ks, v, *vs, n)
+    assert np.allclose(xs, 4.0, atol=1e-15) and np.allclose(v, 4.0, atol=1e-15)
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_pair_sums_vs_long_double(oracle_mod, omesh, level):
+    """Double sums against long-double brute force (same order-free math)."""
+    O = oracle_mod
+    m = omesh(level)
+    z = O.ic_eval(m.fx, rh_n=4, rh_w=0.1, rh_K=0.1)
+    leaves = m.fmask == 0
+    for tx, col, rows in ((m.vx, False, slice(None)), (m.fx, True, leaves)):
+        u = O.velocity(tx, m.fx, z, m.farea, m.fmask, col)[rows]
+        p = O.streamfn(tx, m.fx, z, m.farea, m.fmask, col)[rows]
+        ul, pl = O.velocity_longdouble(tx, m.fx, z, m.farea, m.fmask, col)
+        ul, pl = ul[rows], pl[rows]
+        assert np.abs(u - ul).max() <= 1e-13 * np.abs(ul).max()
+        assert np.abs(p - pl).max() <= 1e-13 * np.abs(pl).max()
+
+
+def test_divided_face_targets_are_singular(oracle_mod, omesh):
+    """A divided face's centre coincides with its centre child's centroid
+    (exactly at level 1), so the reference's collocated sum at divided faces
+    divides by 1 - x.y = 0: those rows are not meaningful and parity is taken
+    over vertices and active faces only."""
+    O = oracle_mod
+    m = omesh(1)
+    z = O.ic_eval(m.fx, rh_n=4, rh_w=0.1, rh_K=0.1)
+    u = O.velocity(m.fx, m.fx, z, m.farea, m.fmask, True)
+    parents = m.fmask == 1
+    assert np.isfinite(u[~parents]).all()
+    assert (~np.isfinite(u[parents])).any()
+
+
+def test_velocity_invariants(oracle_mod, omesh):
+    """u.x = 0 (src/LpmSphereTestKernels.hpp:20-24); rotating every particle
+    rotates the velocity; solid-body rotation zeta = 2 Omega z gives u close to
+    Omega z x x (tests/LpmBVETest.cpp:119-124)."""
+    O = oracle_mod
+    m = omesh(4)
+    Om = 2 * PI
+    z = O.ic_eval(m.fx, sbr=1.0)
+    u = O.velocity(m.vx, m.fx, z, m.farea, m.fmask, False)
+    uf = O.velocity(m.fx, m.fx, z, m.farea, m.fmask, True)
+    assert np.abs(np.einsum("ij,ij->i", u, m.vx)).max() < 1e-12
+    exact = np.stack([-Om * m.vx[:, 1], Om * m.vx[:, 0], 0 * m.vx[:, 0]], axis=1)
+    assert np.abs(u - exact).max() / Om < 5e-3
+    exf = np.stack([-Om * m.fx[:, 1], Om * m.fx[:, 0], 0 * m.fx[:, 0]], axis=1)
+    leaves = m.fmask == 0
+    assert np.abs(uf[leaves] - exf[leaves]).max() / Om < 2e-2
+    # rotation equivariance with a random orthogonal matrix
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    zz = O.ic_eval(m.fx, rh_n=4, rh_w=0.1, rh_K=0.1)
+    u1 = O.velocity(m.vx, m.fx, zz, m.farea, m.fmask, False)
+    u2 = O.velocity(m.vx @ Q.T, m.fx @ Q.T, zz, m.farea, m.fmask, False)
+    assert np.abs(u2 - u1 @ Q.T).max() < 1e-12
+
+
+def test_collocated_self_excluded(oracle_mod, omesh):
+    """Face targets skip j == i (src/LpmBVEKernels.hpp:203,227): the face sum
+    equals the distinct sum with the face's own source masked out."""
+    O = oracle_mod
+    m = omesh(2)
+    z = O.ic_eval(m.fx, rh_n=3, rh_w=0.1, rh_K=0.2)
+    uf = O.velocity(m.fx, m.fx, z, m.farea, m.fmask, True)
+    for i in np.nonzero(m.fmask == 0)[0][:20]:
+        mm = m.fmask.copy()
+        mm[i] = 1
+        ud = O.velocity(m.fx[i:i + 1], m.fx, z, m.farea, mm, False)
+        assert np.abs(ud[0] - uf[i]).max() < 1e-14
+
+
+def test_prng_published_vectors(oracle_mod):
+    """SplitMix64 and xoshiro256** against their published outputs."""
+    import ctypes as C
+    O = oracle_mod.lib()
+    st = C.c_uint64(0)
+    assert O.orc_splitmix64(C.byref(st)) == 0xE220A8397B1DCDAF
+    out = np.zeros(4, np.uint64)
+    O.orc_xoshiro_raw(np.array([1, 2, 3, 4], np.uint64), out, 4)
+    assert list(out) == [11520, 0, 1509978240, 1215971899390074240]
+    uni = np.zeros(20000)
+    nrm = np.zeros(20000)
+    O.orc_rng_draws(12345, uni, 20000, nrm, 20000)
+    assert 0 <= uni.min() and uni.max() < 1 and abs(uni.mean() - 0.5) < 0.01
+    assert abs(nrm.mean()) < 0.03 and abs(nrm.std() - 1) < 0.03
+
+
+def test_rh_closed_form(oracle_mod):
+    """RH wave at a few points against the formula evaluated in Python."""
+    O = oracle_mod
+    pts = np.array([[0.3, 0.4, math.sqrt(1 - 0.25)], [-0.6, 0.0, 0.8], [0.0, -1.0, 0.0]])
+    z = O.ic_eval(pts, rh_n=4, rh_w=0.1, rh_K=0.1)
+    for p, zz in zip(pts, z):
+        lat = math.atan2(p[2], math.hypot(p[0], p[1]))
+        lon = math.atan2(p[1], p[0]) % (2 * math.pi)
+        want = 2 * 0.1 * math.sin(lat) - 0.1 * 30 * math.cos(lat) ** 4 * math.sin(lat) * math.cos(4 * lon)
+        assert abs(zz - want) < 1e-14
+
+
+def test_rk4_sbr_trajectory(oracle_mod, omesh):
+    """Solid-body rotation moves particles rigidly: after n steps the position
+    matches the exact rotation to RK4 and discretisation accuracy
+    (tests/LpmBVETest.cpp:186-191)."""
+    O = oracle_mod
+    m = omesh(3)
+    Om = 2 * PI
+    vz = O.ic_eval(m.vx, sbr=1.0)
+    fz = O.ic_eval(m.fx, sbr=1.0)
+    st = O.State(m, vz, fz)
+    st.init_velocity(0)
+    dt, n = 0.01, 10
+    for _ in range(n):
+        st.step(dt, Om, 0)
+    t = dt * n
+    c, s = math.cos(Om * t), math.sin(Om * t)
+    ex = np.stack([m.vlag[:, 0] * c - m.vlag[:, 1] * s, m.vlag[:, 1] * c + m.vlag[:, 0] * s, m.vlag[:, 2]], 1)
+    assert np.abs(st.vx - ex).max() < 5e-3
+
+
+def test_rk4_face_vorticity_quirk(oracle_mod, omesh):
+    """Under the reference scheme vertex vorticity combines stages (1,2,3,4),
+    the same weights as positions, so zeta + 2 Omega z is conserved to
+    rounding; face vorticity combines (1,2,4,4) (src/LpmBVERK4_Impl.hpp:120-121)
+    and drifts by O(dt).  The Lagrangian scheme conserves it for faces too."""
+    O = oracle_mod
+    m = omesh(2)
+    Om = 2 * PI
+    vz = O.ic_eval(m.vx, rh_n=4, rh_w=0.1, rh_K=0.1)
+    fz = O.ic_eval(m.fx, rh_n=4, rh_w=0.1, rh_K=0.1)
+    a = O.State(m, vz, fz)
+    a.init_velocity(0)
+    a.step(0.05, Om, 0)
+    # vertex vorticity from the absolute-vorticity identity vs ODE: both O(dt^5) close
+    dv = np.abs(a.vzeta - (vz + 2 * Om * (m.vx[:, 2] - a.vx[:, 2]))).max()
+    lv = m.fmask == 0
+    df = np.abs(a.fzeta - (fz + 2 * Om * (m.fx[:, 2] - a.fx[:, 2])))[lv].max()
+    assert df > 10 * dv, (dv, df)
+    b = O.State(m, vz, fz)
+    b.set_lagrangian(Om)
+    b.init_velocity(1)
+    for _ in range(3):
+        b.step(0.05, Om, 1)
+    assert np.abs(b.fzeta + 2 * Om * b.fx[:, 2] - (fz + 2 * Om * m.fx[:, 2]))[lv].max() < 1e-13
+    assert np.abs(np.linalg.norm(b.fx[lv], axis=1) - 1).max() < 1e-15
