@@ -98,6 +98,7 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--targets-per-lane", type=int, default=0)
     ap.add_argument("--splits", type=int, default=0)
     args = ap.parse_args()
@@ -143,33 +144,37 @@ def main() -> int:
     log(f"[bench] pair kernel: {nlaunch} launches, {kernel_ms:.2f} ms each, {achieved:.2f} TFLOP/s")
 
     # FP64 on the same step
-    ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=1)  # warm (capture + entry velocity)
-    ctx.synchronize()
-    t2 = time.perf_counter()
-    ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=args.fp64_steps)
-    ctx.synchronize()
-    t_fp64 = time.perf_counter() - t2
-    rate64 = pairs_step * args.fp64_steps / t_fp64
-    log(f"[bench] FP64: {1e3 * t_fp64 / args.fp64_steps:.1f} ms/step, {rate64:.4e} pairs/s")
+    rate64 = None
+    if args.fp64_steps > 0:
+        ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=1)  # warm (capture + entry velocity)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=args.fp64_steps)
+        ctx.synchronize()
+        t_fp64 = time.perf_counter() - t2
+        rate64 = pairs_step * args.fp64_steps / t_fp64
+        log(f"[bench] FP64: {1e3 * t_fp64 / args.fp64_steps:.1f} ms/step, {rate64:.4e} pairs/s")
 
     # FP32C error against FP64 for one evaluation at the current state
-    ctx.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
-    a = ctx.get_state()
-    ctx.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
-    b = ctx.get_state()
-    u64 = np.concatenate([a["vu"], a["fu"]])
-    u32 = np.concatenate([b["vu"], b["fu"]])
-    p64 = np.concatenate([a["vpsi"], a["fpsi"]])
-    p32 = np.concatenate([b["vpsi"], b["fpsi"]])
-    du = np.linalg.norm(u32 - u64, axis=1)
-    umax = np.linalg.norm(u64, axis=1).max()
-    err = {
-        "velocity_max_rel": float(du.max() / umax),
-        "velocity_rms_rel": float(np.sqrt(np.mean(du ** 2)) / np.sqrt(np.mean(np.sum(u64 ** 2, axis=1)))),
-        "psi_max_rel": float(np.abs(p32 - p64).max() / np.abs(p64).max()),
-        "psi_rms_rel": float(np.sqrt(np.mean((p32 - p64) ** 2) / np.mean(p64 ** 2))),
-    }
-    log(f"[bench] FP32C vs FP64: {err}")
+    err = None
+    if not args.no_accuracy:
+        ctx.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
+        a = ctx.get_state()
+        ctx.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
+        b = ctx.get_state()
+        u64 = np.concatenate([a["vu"], a["fu"]])
+        u32 = np.concatenate([b["vu"], b["fu"]])
+        p64 = np.concatenate([a["vpsi"], a["fpsi"]])
+        p32 = np.concatenate([b["vpsi"], b["fpsi"]])
+        du = np.linalg.norm(u32 - u64, axis=1)
+        umax = np.linalg.norm(u64, axis=1).max()
+        err = {
+            "velocity_max_rel": float(du.max() / umax),
+            "velocity_rms_rel": float(np.sqrt(np.mean(du ** 2)) / np.sqrt(np.mean(np.sum(u64 ** 2, axis=1)))),
+            "psi_max_rel": float(np.abs(p32 - p64).max() / np.abs(p64).max()),
+            "psi_rms_rel": float(np.sqrt(np.mean((p32 - p64) ** 2) / np.mean(p64 ** 2))),
+        }
+        log(f"[bench] FP32C vs FP64: {err}")
     ctx.close()
 
     cpu = None
@@ -200,7 +205,7 @@ def main() -> int:
         },
         "rk4_steps_per_s": 1e3 / ms_step,
         "fp64_pairs_per_s": rate64,
-        "fp32c_vs_fp64_speedup": rate / rate64,
+        "fp32c_vs_fp64_speedup": rate / rate64 if rate64 else None,
         "fp32c_error_vs_fp64": err,
         "roofline": {
             "bound": "valu",

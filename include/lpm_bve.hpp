@@ -1,0 +1,272 @@
+// lpm_bve.hpp -- C++ host interface over the C ABI of lpm_bve.h, in the
+// shape of the reference's BVE classes:
+//
+//   MeshSeed<IcosTriSphereSeed> seed;                       src/LpmMeshSeed.hpp:181-224
+//   seed.setMaxAllocations(nv, ne, nf, depth);             src/LpmMeshSeed.cpp:296-305
+//   BVESphere<IcosTriSphereSeed> sphere(nv, ne, nf);        src/LpmBVESphere.hpp:40
+//   sphere.treeInit(depth, seed);                           src/LpmPolyMesh2d.hpp:171
+//   sphere.set_omega(omega);                                src/LpmBVESphere.cpp:115-124
+//   sphere.init_vorticity(ic);                              src/LpmBVESphere.cpp:126-169
+//   BVERK4 solver(dt, Omega); solver.init(nv, nf);          src/LpmBVERK4.hpp:29-34
+//   solver.advance_timestep(sphere.physVerts.crds, sphere.relVortVerts, sphere.velocityVerts,
+//                           sphere.physFaces.crds, sphere.relVortFaces, sphere.velocityFaces,
+//                           sphere.faces.area, sphere.faces.mask);
+//
+// Fields live in device memory (one lpm_bve_ctx per sphere); the *View members
+// name them, updateHost() fills the host mirrors.  VorticityInitialCondition
+// is the reference's plugin interface (src/LpmVorticityGallery.hpp:13-23):
+// any subclass is evaluated per particle on the host.
+#ifndef LPM_BVE_HPP
+#define LPM_BVE_HPP
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lpm_bve.h"
+
+namespace Lpm {
+
+typedef double Real;
+typedef int64_t Index;
+static constexpr Real PI = 3.1415926535897932384626433832795027975;
+
+inline void check(int rc, const char* what) {
+  if (rc != LPM_OK) throw std::runtime_error(std::string(what) + ": " + lpm_last_error());
+}
+
+struct IcosTriSphereSeed {
+  static constexpr int nverts = 12, nfaces = 20, nedges = 30, nfaceverts = 3;
+  static std::string idString() { return "IcosTriSphereSeed"; }
+};
+
+template <typename SeedType>
+struct MeshSeed {
+  std::string path;  // empty: the library's built-in seed
+  MeshSeed() {}
+  explicit MeshSeed(const std::string& seed_file) : path(seed_file) {}
+  static std::string idString() { return SeedType::idString(); }
+  void setMaxAllocations(Index& nboundary, Index& nedges, Index& nfaces, const int lev) const {
+    check(lpm_icos_alloc(lev, &nboundary, &nedges, &nfaces), "setMaxAllocations");
+  }
+};
+
+struct VorticityInitialCondition {
+  typedef std::shared_ptr<VorticityInitialCondition> ptr;
+  virtual ~VorticityInitialCondition() {}
+  virtual Real eval(const Real& x, const Real& y, const Real& z) const = 0;
+  virtual std::string name() const { return std::string(); }
+  // Evaluate at n points; the gallery fields below route this to the library.
+  virtual void eval_many(const Real* xyz, Index n, Real* zeta) const {
+    for (Index i = 0; i < n; ++i) zeta[i] = eval(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  }
+};
+
+// Composite gallery field evaluated by lpm_ic_eval: solid-body rotation,
+// Rossby-Haurwitz wave, Gaussian vortices.
+struct GalleryVorticity : public VorticityInitialCondition {
+  Real sbr = 0, rh_w = 0, rh_K = 0, sigma = 1;
+  int rh_n = 0;
+  std::vector<Real> centers, amps;
+  Real eval(const Real& x, const Real& y, const Real& z) const override {
+    const Real p[3] = {x, y, z};
+    Real out;
+    eval_many(p, 1, &out);
+    return out;
+  }
+  void eval_many(const Real* xyz, Index n, Real* zeta) const override {
+    check(lpm_ic_eval(xyz, n, zeta, sbr, rh_n, rh_w, rh_K, (int)amps.size(), centers.data(), amps.data(), sigma),
+          "lpm_ic_eval");
+  }
+  void add_vortices(uint64_t seed, int nvort, Real amp_scale, Real sig) {
+    centers.assign(3 * nvort, 0.0);
+    amps.assign(nvort, 0.0);
+    check(lpm_ic_gauss_params(seed, nvort, amp_scale, centers.data(), amps.data()), "lpm_ic_gauss_params");
+    sigma = sig;
+  }
+};
+
+struct SolidBodyRotation : public GalleryVorticity {
+  static constexpr Real OMEGA = 2 * PI;
+  SolidBodyRotation() { sbr = 1.0; }
+  std::string name() const override { return "rotation"; }
+};
+
+struct RossbyHaurwitzWave : public GalleryVorticity {
+  RossbyHaurwitzWave(int n, Real w, Real K) {
+    rh_n = n;
+    rh_w = w;
+    rh_K = K;
+  }
+  std::string name() const override { return "rossby_haurwitz"; }
+  Real phase_speed(Real omega) const { return (rh_n * (rh_n + 3) * rh_w - 2 * omega) / ((rh_n + 1) * (rh_n + 2)); }
+};
+
+template <typename SeedType>
+class BVESphere;
+
+// Names one field of a sphere's device-resident state.
+template <typename SeedType>
+struct FieldView {
+  BVESphere<SeedType>* sphere;
+  int id;
+};
+
+template <typename SeedType>
+class BVESphere {
+ public:
+  enum { VX, VZETA, VU, FX, FZETA, FU, FAREA, FMASK };
+  struct CoordsView {
+    FieldView<SeedType> crds;
+  };
+  struct FacesView {
+    FieldView<SeedType> area, mask;
+  };
+
+  CoordsView physVerts, physFaces;
+  FacesView faces;
+  FieldView<SeedType> relVortVerts, relVortFaces, velocityVerts, velocityFaces;
+  Real Omega = 2 * PI;
+  Real t = 0;
+
+  // host mirrors (filled by treeInit / init_vorticity / updateHost)
+  std::vector<Real> vert_x, vert_lag, face_x, face_lag, face_area;
+  std::vector<uint8_t> face_mask;
+  std::vector<Real> rel_vort_verts, rel_vort_faces, abs_vort_verts, abs_vort_faces;
+  std::vector<Real> vel_verts, vel_faces, psi_verts, psi_faces;
+
+  BVESphere(Index nmaxverts, Index nmaxedges, Index nmaxfaces, int nq = 0, int device = 0)
+      : nmaxv_(nmaxverts), nmaxf_(nmaxfaces) {
+    (void)nmaxedges;
+    (void)nq;
+    physVerts.crds = {this, VX};
+    physFaces.crds = {this, FX};
+    faces.area = {this, FAREA};
+    faces.mask = {this, FMASK};
+    relVortVerts = {this, VZETA};
+    relVortFaces = {this, FZETA};
+    velocityVerts = {this, VU};
+    velocityFaces = {this, FU};
+    check(lpm_bve_create(device, &ctx_), "lpm_bve_create");
+  }
+  ~BVESphere() { lpm_bve_destroy(ctx_); }
+  BVESphere(const BVESphere&) = delete;
+  BVESphere& operator=(const BVESphere&) = delete;
+
+  void treeInit(int depth, const MeshSeed<SeedType>& seed) {
+    Index nv, ne, nf;
+    seed.setMaxAllocations(nv, ne, nf, depth);
+    if (nv > nmaxv_ || nf > nmaxf_) throw std::runtime_error("treeInit: not enough memory");
+    vert_x.assign(3 * nv, 0);
+    vert_lag.assign(3 * nv, 0);
+    face_x.assign(3 * nf, 0);
+    face_lag.assign(3 * nf, 0);
+    face_area.assign(nf, 0);
+    face_mask.assign(nf, 0);
+    int64_t counts[4];
+    check(lpm_icos_tree_init(depth, seed.path.empty() ? nullptr : seed.path.c_str(), vert_x.data(),
+                             vert_lag.data(), face_x.data(), face_lag.data(), face_area.data(), face_mask.data(),
+                             nullptr, nullptr, counts),
+          "treeInit");
+    nverts_ = counts[0];
+    nfaces_ = counts[2];
+    nleaves_ = counts[3];
+  }
+
+  Index nvertsHost() const { return nverts_; }
+  Index nfacesHost() const { return nfaces_; }
+  Index nleavesHost() const { return nleaves_; }
+
+  void set_omega(const Real& omg) {
+    if (omg_set_) return;
+    Omega = omg;
+    omg_set_ = true;
+  }
+
+  // Host IC evaluation, absolute vorticity, upload, then the device solves
+  // for stream function and velocity (src/LpmBVESphere.cpp:126-169).
+  void init_vorticity(const VorticityInitialCondition::ptr relvort, int prec = LPM_PREC_FP64,
+                      int scheme = LPM_SCHEME_REF) {
+    rel_vort_verts.assign(nverts_, 0);
+    rel_vort_faces.assign(nfaces_, 0);
+    relvort->eval_many(vert_x.data(), nverts_, rel_vort_verts.data());
+    relvort->eval_many(face_x.data(), nfaces_, rel_vort_faces.data());
+    abs_vort_verts.resize(nverts_);
+    abs_vort_faces.resize(nfaces_);
+    for (Index i = 0; i < nverts_; ++i) abs_vort_verts[i] = rel_vort_verts[i] + 2 * Omega * vert_x[3 * i + 2];
+    for (Index i = 0; i < nfaces_; ++i) abs_vort_faces[i] = rel_vort_faces[i] + 2 * Omega * face_x[3 * i + 2];
+    check(lpm_bve_set_state(ctx_, nverts_, vert_x.data(), rel_vort_verts.data(), nfaces_, face_x.data(),
+                            rel_vort_faces.data(), face_area.data(), face_mask.data()),
+          "lpm_bve_set_state");
+    check(lpm_bve_solve(ctx_, prec, scheme, 1), "lpm_bve_solve");
+  }
+
+  // Stream function at the current state (the post-step solve of
+  // tests/LpmBVETest.cpp:155-159).
+  void solve_streamfn(int prec = LPM_PREC_FP64, int scheme = LPM_SCHEME_REF) {
+    check(lpm_bve_solve(ctx_, prec, scheme, 1), "lpm_bve_solve");
+  }
+
+  void updateHost() {
+    vel_verts.resize(3 * nverts_);
+    vel_faces.resize(3 * nfaces_);
+    psi_verts.resize(nverts_);
+    psi_faces.resize(nfaces_);
+    check(lpm_bve_get_state(ctx_, vert_x.data(), rel_vort_verts.data(), vel_verts.data(), psi_verts.data(),
+                            face_x.data(), rel_vort_faces.data(), vel_faces.data(), psi_faces.data()),
+          "lpm_bve_get_state");
+  }
+
+  Real avg_mesh_size_radians() const { return std::sqrt(4 * PI / nfaces_); }
+  Real avg_mesh_size_degrees() const { return avg_mesh_size_radians() * 180.0 / PI; }
+  Real circulation() {
+    Real c;
+    check(lpm_bve_circulation(ctx_, &c), "lpm_bve_circulation");
+    return c;
+  }
+  lpm_bve_ctx* context() { return ctx_; }
+
+ private:
+  lpm_bve_ctx* ctx_ = nullptr;
+  Index nmaxv_, nmaxf_;
+  Index nverts_ = 0, nfaces_ = 0, nleaves_ = 0;
+  bool omg_set_ = false;
+};
+
+class BVERK4 {
+ public:
+  Real dt, Omega;
+  Index nverts = 0, nfaces = 0;
+  int scheme = LPM_SCHEME_REF, prec = LPM_PREC_FP64;
+  bool use_graph = true;
+
+  BVERK4(const Real& timestep, const Real& omg) : dt(timestep), Omega(omg) {}
+  void init(const Index& nv, const Index& nf) {
+    nverts = nv;
+    nfaces = nf;
+  }
+
+  template <typename SeedType>
+  void advance_timestep(FieldView<SeedType> vx, FieldView<SeedType> vzeta, FieldView<SeedType> vvel,
+                        FieldView<SeedType> fx, FieldView<SeedType> fzeta, FieldView<SeedType> fvel,
+                        FieldView<SeedType> fa, FieldView<SeedType> fm, int nsteps = 1) {
+    typedef BVESphere<SeedType> S;
+    BVESphere<SeedType>* s = vx.sphere;
+    const FieldView<SeedType> v[8] = {vx, vzeta, vvel, fx, fzeta, fvel, fa, fm};
+    const int ids[8] = {S::VX, S::VZETA, S::VU, S::FX, S::FZETA, S::FU, S::FAREA, S::FMASK};
+    for (int k = 0; k < 8; ++k)
+      if (v[k].sphere != s || v[k].id != ids[k])
+        throw std::invalid_argument("advance_timestep: views must be one sphere's fields, in order");
+    if (s->nvertsHost() != nverts || s->nfacesHost() != nfaces)
+      throw std::invalid_argument("advance_timestep: init(nverts, nfaces) does not match the sphere");
+    check(lpm_bve_advance(s->context(), dt, Omega, scheme, prec, nsteps, use_graph ? 1 : 0), "lpm_bve_advance");
+    s->t += nsteps * dt;
+  }
+};
+
+}  // namespace Lpm
+
+#endif  // LPM_BVE_HPP

@@ -1,0 +1,200 @@
+// bve_driver: the reference's BVE test driver (tests/LpmBVETest.cpp:35-296)
+// on the MI355X library, through the C++ interface of include/lpm_bve.hpp.
+//
+//   bve_driver [-d depth] [-dt dt] [-tf tfinal] [-ic sbr|rh] [-scheme ref|lagr]
+//              [-prec fp64|fp32c] [-f output_interval]
+//
+// Solid-body rotation (default) has an exact solution: velocity Omega z x x
+// and rigid rotation of the Lagrangian coordinates; the driver prints the
+// area-weighted error norms (src/LpmErrorNorms.cpp:26-45) at t = 0 and at
+// tfinal and, unlike the reference, fails (exit 1) when they exceed bounds.
+// The Rossby-Haurwitz case reports the measured vs analytic pattern speed.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lpm_bve.hpp"
+
+using namespace Lpm;
+
+namespace {
+
+struct Norms {
+  Real l1, l2, linf;
+};
+
+// Vector error norms with area weights over active faces.
+Norms vector_norms(const std::vector<Real>& appx, const std::vector<Real>& exact, const std::vector<Real>& wt,
+                   const std::vector<uint8_t>& mask) {
+  Real l1n = 0, l1d = 0, l2n = 0, l2d = 0, lin = 0, lid = 0;
+  for (size_t i = 0; i < wt.size(); ++i) {
+    if (mask[i]) continue;
+    Real e2 = 0, x2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      const Real d = appx[3 * i + k] - exact[3 * i + k];
+      e2 += d * d;
+      x2 += exact[3 * i + k] * exact[3 * i + k];
+    }
+    const Real em = std::sqrt(e2), xm = std::sqrt(x2);
+    l1n += em * wt[i];
+    l1d += xm * wt[i];
+    l2n += e2 * wt[i];
+    l2d += x2 * wt[i];
+    lin = std::max(lin, em);
+    lid = std::max(lid, xm);
+  }
+  return {l1n / l1d, std::sqrt(l2n / l2d), lin / lid};
+}
+
+void print_norms(const char* label, const Norms& n) {
+  std::printf("%s ErrNorms: l1 = %g l2 = %g linf = %g\n", label, n.l1, n.l2, n.linf);
+}
+
+// Phase of the wavenumber-n longitude mode of vorticity near latitude lat0.
+Real rh_phase(const std::vector<Real>& x, const std::vector<Real>& zeta, const std::vector<uint8_t>& mask, int n,
+              Real lat0) {
+  Real c = 0, s = 0;
+  for (size_t i = 0; i < zeta.size(); ++i) {
+    if (mask[i]) continue;
+    const Real* p = &x[3 * i];
+    const Real lat = std::atan2(p[2], std::sqrt(p[0] * p[0] + p[1] * p[1]));
+    if (std::fabs(lat - lat0) > 0.15) continue;
+    const Real lon = std::atan2(p[1], p[0]);
+    c += zeta[i] * std::cos(n * lon);
+    s += zeta[i] * std::sin(n * lon);
+  }
+  return std::atan2(s, c) / n;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  int depth = 3, output_interval = 0;
+  Real dt = 0.01, tfinal = 1.0;
+  std::string ic_name = "sbr";
+  int scheme = LPM_SCHEME_REF, prec = LPM_PREC_FP64;
+  for (int i = 1; i < argc; ++i) {
+    const std::string t = argv[i];
+    if (t == "-d" && i + 1 < argc) depth = std::stoi(argv[++i]);
+    else if (t == "-dt" && i + 1 < argc) dt = std::stod(argv[++i]);
+    else if (t == "-tf" && i + 1 < argc) tfinal = std::stod(argv[++i]);
+    else if (t == "-f" && i + 1 < argc) output_interval = std::stoi(argv[++i]);
+    else if (t == "-ic" && i + 1 < argc) ic_name = argv[++i];
+    else if (t == "-scheme" && i + 1 < argc) scheme = std::string(argv[++i]) == "lagr" ? LPM_SCHEME_LAGR : LPM_SCHEME_REF;
+    else if (t == "-prec" && i + 1 < argc) prec = std::string(argv[++i]) == "fp32c" ? LPM_PREC_FP32C : LPM_PREC_FP64;
+  }
+  if (prec == LPM_PREC_FP32C) scheme = LPM_SCHEME_LAGR;
+  try {
+    MeshSeed<IcosTriSphereSeed> seed;
+    Index nmaxverts, nmaxedges, nmaxfaces;
+    seed.setMaxAllocations(nmaxverts, nmaxedges, nmaxfaces, depth);
+    BVESphere<IcosTriSphereSeed> sphere(nmaxverts, nmaxedges, nmaxfaces, 0);
+    sphere.treeInit(depth, seed);
+    const Real Omega = 2 * PI;
+    std::shared_ptr<VorticityInitialCondition> relvort;
+    std::shared_ptr<RossbyHaurwitzWave> rh;
+    if (ic_name == "rh") {
+      rh = std::make_shared<RossbyHaurwitzWave>(4, 0.1, 0.1);
+      relvort = rh;
+    } else {
+      relvort = std::make_shared<SolidBodyRotation>();
+      sphere.set_omega(0);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    sphere.init_vorticity(relvort, prec, scheme);
+    const int ntimesteps = (int)std::floor(tfinal / dt);
+    dt = tfinal / ntimesteps;
+    BVERK4 solver(dt, Omega);
+    solver.scheme = scheme;
+    solver.prec = prec;
+    solver.init(sphere.nvertsHost(), sphere.nfacesHost());
+    std::printf("BVE driver: %s, depth %d (%lld verts, %lld faces, %lld leaves), mesh size %g deg\n",
+                relvort->name().c_str(), depth, (long long)sphere.nvertsHost(), (long long)sphere.nfacesHost(),
+                (long long)sphere.nleavesHost(), sphere.avg_mesh_size_degrees());
+    std::printf("\tdt = %g, tfinal = %g, scheme = %s, prec = %s\n", dt, tfinal,
+                scheme == LPM_SCHEME_LAGR ? "lagr" : "ref", prec == LPM_PREC_FP32C ? "fp32c" : "fp64");
+    sphere.updateHost();
+    const std::vector<Real> lag_faces = sphere.face_lag;
+    const std::vector<Real> z0 = sphere.rel_vort_faces;
+    auto exact_velocity = [&](const std::vector<Real>& x) {
+      std::vector<Real> u(x.size());
+      for (size_t i = 0; i < x.size() / 3; ++i) {
+        u[3 * i] = -Omega * x[3 * i + 1];
+        u[3 * i + 1] = Omega * x[3 * i];
+        u[3 * i + 2] = 0.0;
+      }
+      return u;
+    };
+    bool ok = true;
+    Real phase0 = 0;
+    if (!rh) {
+      const Norms n0 = vector_norms(sphere.vel_faces, exact_velocity(sphere.face_x), sphere.face_area, sphere.face_mask);
+      print_norms("velocity error at t=0", n0);
+      ok = ok && n0.linf < 0.05;
+    } else {
+      phase0 = rh_phase(sphere.face_x, sphere.rel_vort_faces, sphere.face_mask, 4, 0.6);
+    }
+    int steps_done = 0;
+    const int chunk = output_interval > 0 ? output_interval : ntimesteps;
+    while (steps_done < ntimesteps) {
+      const int n = std::min(chunk, ntimesteps - steps_done);
+      solver.advance_timestep(sphere.physVerts.crds, sphere.relVortVerts, sphere.velocityVerts,
+                              sphere.physFaces.crds, sphere.relVortFaces, sphere.velocityFaces, sphere.faces.area,
+                              sphere.faces.mask, n);
+      steps_done += n;
+      if (output_interval > 0) {
+        sphere.updateHost();
+        std::printf("\tstep %d, t = %g, circulation = %.3e\n", steps_done, sphere.t, sphere.circulation());
+      }
+    }
+    sphere.solve_streamfn(prec, scheme);
+    sphere.updateHost();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    Real tangent = 0;
+    for (Index i = 0; i < sphere.nfacesHost(); ++i) {
+      if (sphere.face_mask[i]) continue;
+      Real d = 0;
+      for (int k = 0; k < 3; ++k) d += sphere.face_x[3 * i + k] * sphere.vel_faces[3 * i + k];
+      tangent = std::max(tangent, std::fabs(d));
+    }
+    std::printf("\tmax |u.x| at faces = %g\n", tangent);
+    if (!rh) {
+      const Real t = sphere.t, c = std::cos(Omega * t), s = std::sin(Omega * t);
+      std::vector<Real> exact_pos(lag_faces.size());
+      for (size_t i = 0; i < lag_faces.size() / 3; ++i) {
+        const Real* a = &lag_faces[3 * i];
+        exact_pos[3 * i] = a[0] * c - a[1] * s;
+        exact_pos[3 * i + 1] = a[1] * c + a[0] * s;
+        exact_pos[3 * i + 2] = a[2];
+      }
+      const Norms nv = vector_norms(sphere.vel_faces, exact_velocity(sphere.face_x), sphere.face_area, sphere.face_mask);
+      const Norms np = vector_norms(sphere.face_x, exact_pos, sphere.face_area, sphere.face_mask);
+      Real vort = 0;
+      for (Index i = 0; i < sphere.nfacesHost(); ++i)
+        if (!sphere.face_mask[i]) vort = std::max(vort, std::fabs(sphere.rel_vort_faces[i] - z0[i]));
+      print_norms("velocity error at t=tfinal", nv);
+      print_norms("position error at t=tfinal", np);
+      std::printf("vorticity change at t=tfinal: linf = %g\n", vort);
+      ok = ok && nv.linf < 0.05 && np.linf < 0.01;
+    } else {
+      const Real phase1 = rh_phase(sphere.face_x, sphere.rel_vort_faces, sphere.face_mask, 4, 0.6);
+      Real dphi = phase1 - phase0;
+      while (dphi > PI / 4) dphi -= PI / 2;
+      while (dphi < -PI / 4) dphi += PI / 2;
+      const Real nu = rh->phase_speed(Omega);
+      std::printf("RH wave pattern speed: measured %.6f rad/time, analytic %.6f (%.2f%% off)\n", dphi / sphere.t, nu,
+                  100.0 * std::fabs(dphi / sphere.t - nu) / std::fabs(nu));
+      ok = ok && std::fabs(dphi / sphere.t - nu) < 0.05 * std::fabs(nu);
+    }
+    ok = ok && tangent < 1e-6;
+    std::printf("elapsed %.3f s (%d steps)\n", secs, ntimesteps);
+    std::printf(ok ? "tests pass\n" : "tests FAIL\n");
+    return ok ? 0 : 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "bve_driver: %s\n", e.what());
+    return 2;
+  }
+}

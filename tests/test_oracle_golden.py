@@ -286,3 +286,26 @@ def test_rk4_face_vorticity_quirk(oracle_mod, omesh):
         b.step(0.05, Om, 1)
     assert np.abs(b.fzeta + 2 * Om * b.fx[:, 2] - (fz + 2 * Om * m.fx[:, 2]))[lv].max() < 1e-13
     assert np.abs(np.linalg.norm(b.fx[lv], axis=1) - 1).max() < 1e-15
+
+
+def test_reference_scheme_sbr_instability(oracle_mod, omesh):
+    """Finding: the reference step (no renormalisation, literal 1 - x.y)
+    diverges for solid-body rotation at level 4 with dt = 0.01: stage
+    positions x + k/2 lie (dt |u| / 2)^2 ~ 1e-3 off the sphere, which exceeds
+    1 - x.y of nearest neighbours (~1.2e-3).  The renormalised Lagrangian
+    scheme stays on the exact rotation."""
+    O = oracle_mod
+    from helpers import LeafMesh
+    lm = LeafMesh(omesh(4))
+    vz, fz = O.ic_eval(lm.vx, sbr=1.0), O.ic_eval(lm.fx, sbr=1.0)
+    errs = []
+    for scheme in (0, 1):
+        st = O.State(lm, vz, fz)
+        st.set_lagrangian(2 * PI)
+        st.init_velocity(scheme)
+        for _ in range(20):
+            st.step(0.01, 2 * PI, scheme)
+        c, s = math.cos(2 * PI * 0.2), math.sin(2 * PI * 0.2)
+        ex = np.stack([lm.vx[:, 0] * c - lm.vx[:, 1] * s, lm.vx[:, 1] * c + lm.vx[:, 0] * s, lm.vx[:, 2]], 1)
+        errs.append(np.abs(st.vx - ex).max())
+    assert errs[0] > 1.0 and errs[1] < 1e-2, errs
