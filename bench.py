@@ -101,6 +101,8 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--targets-per-lane", type=int, default=0)
     ap.add_argument("--splits", type=int, default=0)
+    ap.add_argument("--near-factor", type=float, default=None)
+    ap.add_argument("--staging", choices=["lds", "scalar"], default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the multi-GPU sharding is a design "
@@ -121,6 +123,10 @@ def main() -> int:
     ctx.set_state(mesh.vx, vz, fx, fz, farea, np.zeros(nl, np.uint8))
     if args.targets_per_lane:
         ctx.set_launch(args.targets_per_lane, args.splits)
+    if args.near_factor is not None:
+        ctx.set_near_factor(args.near_factor)
+    if args.staging:
+        ctx.set_source_staging(args.staging == "lds")
 
     # FP32C: warmup, then K timed steps replayed from the captured graph
     ctx.solve(L.PREC_FP32C, L.SCHEME_LAGR)

@@ -161,6 +161,17 @@ int lpm_bve_sizes(lpm_bve_ctx* ctx, int64_t* sizes4);
  * (0 = automatic).  Applies to later solves and steps. */
 int lpm_bve_set_launch(lpm_bve_ctx* ctx, int targets_per_lane, int source_splits);
 
+/* FP32C: source tiles whose centre lies within `factor` tile radii of a
+ * target use the explicit |y - x|^2 form, farther tiles the expanded
+ * |D|^2 + |d|^2 + 2 D.d form (fewer operations; its cancellation is bounded by
+ * ((factor+1)/(factor-1))^2).  Default 4; a tile holding a target's own
+ * source always uses the explicit form. */
+int lpm_bve_set_near_factor(lpm_bve_ctx* ctx, double factor);
+
+/* FP32C: stage source tiles through LDS (1, default) or read them through
+ * the scalar cache (0). */
+int lpm_bve_set_source_staging(lpm_bve_ctx* ctx, int lds);
+
 /* The context's HIP stream (hipStream_t) and device id. */
 int lpm_bve_stream(lpm_bve_ctx* ctx, void** stream, int* device);
 

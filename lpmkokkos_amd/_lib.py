@@ -53,6 +53,8 @@ SIGNATURES = {
     "lpm_bve_synchronize": [C.c_void_p],
     "lpm_bve_sizes": [C.c_void_p, _i64],
     "lpm_bve_set_launch": [C.c_void_p, C.c_int, C.c_int],
+    "lpm_bve_set_near_factor": [C.c_void_p, C.c_double],
+    "lpm_bve_set_source_staging": [C.c_void_p, C.c_int],
     "lpm_bve_stream": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)],
     "lpm_bve_last_pair_timing": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "lpm_bve_enable_timing": [C.c_void_p, C.c_int],

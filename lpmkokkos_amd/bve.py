@@ -206,6 +206,12 @@ class Context:
     def set_launch(self, targets_per_lane: int, source_splits: int = 0):
         call("lpm_bve_set_launch", self.h, targets_per_lane, source_splits)
 
+    def set_source_staging(self, lds: bool):
+        call("lpm_bve_set_source_staging", self.h, int(lds))
+
+    def set_near_factor(self, factor: float):
+        call("lpm_bve_set_near_factor", self.h, factor)
+
     def enable_timing(self, on: bool):
         call("lpm_bve_enable_timing", self.h, int(on))
 

@@ -64,16 +64,28 @@ __global__ __launch_bounds__(kBlock) void k_prep_f64(double4* __restrict__ src, 
   }
 }
 
-// FP32C source records: one workgroup per tile computes the tile centre
-// (FP64 mean of its sources) and the tile's FP64 sum of q, then stores each
-// source as {fl32(y - c), fl32(q)} with q = (-zeta A)/(2 pi).  Padding
-// records sit 8 units away with q = 0.
-__global__ __launch_bounds__(kTile) void k_prep_f32c(float4* __restrict__ src, double4* __restrict__ ctr,
-                                                     double* __restrict__ srcq, Pos3 pos,
+// FP32C source record: a = {dx, dy, dz, |d|^2} with d = fl32(y - c) the
+// offset from the tile centre c; b = {q, q dx, q dy, q dz} with
+// q = fl32((-zeta A)/(2 pi)).  |d|^2 and q d are formed in FP32 from the
+// stored offsets so that both pair forms see the same source position.
+struct Src32 {
+  float4 a, b;
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// One workgroup per tile: tile centre (FP64 mean of its sources), the
+// tile's FP64 sum of q, its radius max |d| (rounded up), and the records.
+// Padding records sit 8 units from the centre with q = 0.
+__global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, double4* __restrict__ ctr,
+                                                     float* __restrict__ trad, double* __restrict__ srcq, Pos3 pos,
                                                      const double* __restrict__ zeta,
                                                      const double* __restrict__ area,
                                                      const int* __restrict__ smap, int ns) {
-  __shared__ double red[4][kTile / 64];
+  __shared__ double red[5][kTile / 64];
   const int t = blockIdx.x;
   const int j = t * kTile + threadIdx.x;
   const bool valid = j < ns;
@@ -101,10 +113,30 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(float4* __restrict__ src, d
   c[0] *= inv;
   c[1] *= inv;
   c[2] *= inv;
-  if (threadIdx.x == 0) ctr[t] = make_double4(c[0], c[1], c[2], c[3]);
-  src[j] = valid ? make_float4((float)(p0 - c[0]), (float)(p1 - c[1]), (float)(p2 - c[2]), (float)q)
-                 : make_float4(8.0f, 8.0f, 8.0f, 0.0f);
+  Src32 r;
+  double rad = 0.0;
+  if (valid) {
+    const float dx = (float)(p0 - c[0]), dy = (float)(p1 - c[1]), dz = (float)(p2 - c[2]);
+    const float qf = (float)q;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    r.a = make_float4(dx, dy, dz, d2);
+    r.b = make_float4(qf, qf * dx, qf * dy, qf * dz);
+    rad = sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+  } else {
+    r.a = make_float4(8.0f, 8.0f, 8.0f, 192.0f);
+    r.b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  src[j] = r;
   srcq[j] = q;
+  rad = wave_max(rad);
+  __syncthreads();
+  if (lane == 0) red[4][wave] = rad;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ctr[t] = make_double4(c[0], c[1], c[2], c[3]);
+    const double rr = fmax(fmax(red[4][0], red[4][1]), fmax(red[4][2], red[4][3]));
+    trad[t] = (float)(rr * (1.0 + 1e-6));
+  }
 }
 
 // Sum of per-tile q sums in tile order (one workgroup; deterministic).
@@ -131,6 +163,7 @@ struct PairArgs {
   int tiles_per_split;   // tiles per grid.y slice
   double* part;          // [splits][kNC][nt_pad]
   int nt_pad;
+  float near_factor;  // FP32C: near-form threshold in tile radii
 };
 
 template <int R>
@@ -230,26 +263,51 @@ __global__ __launch_bounds__(kBlock) void k_pair_f64(const double4* __restrict__
   store_partials<R>(a, idx, S);
 }
 
-// One FP32 pair term from tile-local offsets; P accumulates q log2(|d|^2).
+// Near form: d = D + dx formed explicitly, f = q / |d|^2, S += f d.
+// P accumulates q log2(|d|^2).  `me` drops the target's own source.
 template <bool PSI, bool CHECK>
-__device__ __forceinline__ void pair32(const float (&D)[3], const float4 s, bool me, float (&S)[kNC]) {
-  const float e0 = D[0] + s.x, e1 = D[1] + s.y, e2 = D[2] + s.z;
+__device__ __forceinline__ void pair32_near(const float (&D)[3], const Src32& s, bool me, float (&T)[kNC]) {
+  const float e0 = D[0] + s.a.x, e1 = D[1] + s.a.y, e2 = D[2] + s.a.z;
   float r2 = fmaf(e0, e0, fmaf(e1, e1, e2 * e2));
-  float q = s.w;
+  float q = s.b.x;
   if (CHECK && me) {
     r2 = 1.0f;
     q = 0.0f;
   }
   const float f = q * __builtin_amdgcn_rcpf(r2);
-  S[0] = fmaf(f, e0, S[0]);
-  S[1] = fmaf(f, e1, S[1]);
-  S[2] = fmaf(f, e2, S[2]);
-  if (PSI) S[3] = fmaf(q, __builtin_amdgcn_logf(r2), S[3]);
+  T[0] = fmaf(f, e0, T[0]);
+  T[1] = fmaf(f, e1, T[1]);
+  T[2] = fmaf(f, e2, T[2]);
+  if (PSI) T[3] = fmaf(q, __builtin_amdgcn_logf(r2), T[3]);
 }
 
+// Far form: |d|^2 = |D|^2 + |dx|^2 + 2 D.dx from the precomputed |dx|^2;
+// accumulates F = sum q/|d|^2 and G = sum q dx/|d|^2, so that the tile's
+// sum of q d/|d|^2 is D F + G.  8 VALU operations per pair.
+template <bool PSI>
+__device__ __forceinline__ void pair32_far(const float (&D2)[3], float E, const Src32& s, float (&F)[kNC + 1]) {
+  float r2 = E + s.a.w;
+  r2 = fmaf(D2[0], s.a.x, r2);
+  r2 = fmaf(D2[1], s.a.y, r2);
+  r2 = fmaf(D2[2], s.a.z, r2);
+  const float inv = __builtin_amdgcn_rcpf(r2);
+  F[0] = fmaf(inv, s.b.x, F[0]);
+  F[1] = fmaf(inv, s.b.y, F[1]);
+  F[2] = fmaf(inv, s.b.z, F[2]);
+  F[3] = fmaf(inv, s.b.w, F[3]);
+  if (PSI) F[4] = fmaf(s.b.x, __builtin_amdgcn_logf(r2), F[4]);
+}
+
+// Tiles whose centre lies within `near_factor` tile radii of a target of
+// the wave use the near form, and so does the tile holding a target's own
+// source; the far form's cancellation in |d|^2 is bounded by
+// ((near_factor + 1) / (near_factor - 1))^2.  Default 4.
+constexpr float kNearDefault = 4.0f;
+
 template <int R, bool PSI>
-__global__ __launch_bounds__(kBlock) void k_pair_f32c(const float4* __restrict__ src,
-                                                      const double4* __restrict__ ctr, PairArgs a) {
+__global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ src,
+                                                      const double4* __restrict__ ctr,
+                                                      const float* __restrict__ trad, PairArgs a) {
   double x[R][3];
   int self[R], idx[R];
   load_targets<R>(a, x, self, idx);
@@ -262,32 +320,161 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const float4* __restrict__
   for (int t = t0; t < t1; ++t) {
     const int base = t * kTile;
     const double4 c = ctr[t];
-    float D[R][3], T[R][kNC];
+    const float rn = a.near_factor * trad[t];
+    float D[R][3], E[R];
+    int near = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       D[r][0] = (float)(c.x - x[r][0]);
       D[r][1] = (float)(c.y - x[r][1]);
       D[r][2] = (float)(c.z - x[r][2]);
-      for (int k = 0; k < kNC; ++k) T[r][k] = 0.0f;
+      E[r] = fmaf(D[r][0], D[r][0], fmaf(D[r][1], D[r][1], D[r][2] * D[r][2]));
+      near |= E[r] < rn * rn;
     }
-    const float4* ts = src + base;
-    if (!wave_has_self<R>(self, base)) {
-#pragma unroll 8
-      for (int j = 0; j < kTile; ++j) {
-        const float4 s = ts[j];
+    const Src32* ts = src + base;
+    float T[R][kNC];
+    const bool has_self = wave_has_self<R>(self, base);
+    if (has_self || __ballot(near) != 0) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) pair32<PSI, false>(D[r], s, false, T[r]);
+      for (int r = 0; r < R; ++r)
+        for (int k = 0; k < kNC; ++k) T[r][k] = 0.0f;
+      if (!has_self) {
+#pragma unroll 4
+        for (int j = 0; j < kTile; ++j) {
+          const Src32 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) pair32_near<PSI, false>(D[r], sj, false, T[r]);
+        }
+      } else {
+        for (int j = 0; j < kTile; ++j) {
+          const Src32 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) pair32_near<PSI, true>(D[r], sj, base + j == self[r], T[r]);
+        }
       }
     } else {
-      for (int j = 0; j < kTile; ++j) {
-        const float4 s = ts[j];
+      float D2[R][3], F[R][kNC + 1];
 #pragma unroll
-        for (int r = 0; r < R; ++r) pair32<PSI, true>(D[r], s, base + j == self[r], T[r]);
+      for (int r = 0; r < R; ++r) {
+        for (int k = 0; k < 3; ++k) D2[r][k] = 2.0f * D[r][k];
+        for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0f;
+      }
+      for (int j = 0; j < kTile; j += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const Src32 sj = ts[j + u];
+#pragma unroll
+          for (int r = 0; r < R; ++r) pair32_far<PSI>(D2[r], E[r], sj, F[r]);
+        }
+        // Spread the reciprocals through the iteration: the transcendental
+        // unit runs beside the VALU, but a run of back-to-back v_rcp stalls
+        // the issuing wave.  Pattern per pair: 4 VALU, 1 TRANS, 4 VALU.
+#pragma unroll
+        for (int k = 0; k < 4 * R; ++k) {
+          __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+          __builtin_amdgcn_sched_group_barrier(0x400, PSI ? 2 : 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T[r][0] = fmaf(D[r][0], F[r][0], F[r][1]);
+        T[r][1] = fmaf(D[r][1], F[r][0], F[r][2]);
+        T[r][2] = fmaf(D[r][2], F[r][0], F[r][3]);
+        T[r][3] = F[r][4];
       }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
       for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
+  }
+  store_partials<R>(a, idx, S);
+}
+
+// Same arithmetic as k_pair_f32c with the source tiles staged through LDS:
+// the workgroup copies tile t+1 (coalesced, 8 KB) while it works on tile t,
+// and waves read records with uniform-address LDS loads into VGPRs.
+template <int R, bool PSI>
+__global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restrict__ src,
+                                                          const double4* __restrict__ ctr,
+                                                          const float* __restrict__ trad, PairArgs a) {
+  __shared__ Src32 sh[2][kTile];
+  double x[R][3];
+  int self[R], idx[R];
+  load_targets<R>(a, x, self, idx);
+  double S[R][kNC];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
+  const int t0 = blockIdx.y * a.tiles_per_split;
+  const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
+  if (t0 < t1) sh[0][threadIdx.x] = src[(long)t0 * kTile + threadIdx.x];
+  __syncthreads();
+  for (int t = t0; t < t1; ++t) {
+    const int buf = (t - t0) & 1;
+    const bool more = t + 1 < t1;
+    Src32 nxt;
+    if (more) nxt = src[(long)(t + 1) * kTile + threadIdx.x];
+    const int base = t * kTile;
+    const double4 c = ctr[t];
+    const float rn = a.near_factor * trad[t];
+    float D[R][3], E[R];
+    int near = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      D[r][0] = (float)(c.x - x[r][0]);
+      D[r][1] = (float)(c.y - x[r][1]);
+      D[r][2] = (float)(c.z - x[r][2]);
+      E[r] = fmaf(D[r][0], D[r][0], fmaf(D[r][1], D[r][1], D[r][2] * D[r][2]));
+      near |= E[r] < rn * rn;
+    }
+    const Src32* ts = sh[buf];
+    float T[R][kNC];
+    const bool has_self = wave_has_self<R>(self, base);
+    if (has_self || __ballot(near) != 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        for (int k = 0; k < kNC; ++k) T[r][k] = 0.0f;
+      if (!has_self) {
+#pragma unroll 4
+        for (int j = 0; j < kTile; ++j) {
+          const Src32 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) pair32_near<PSI, false>(D[r], sj, false, T[r]);
+        }
+      } else {
+        for (int j = 0; j < kTile; ++j) {
+          const Src32 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) pair32_near<PSI, true>(D[r], sj, base + j == self[r], T[r]);
+        }
+      }
+    } else {
+      float D2[R][3], F[R][kNC + 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        for (int k = 0; k < 3; ++k) D2[r][k] = 2.0f * D[r][k];
+        for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0f;
+      }
+#pragma unroll 4
+      for (int j = 0; j < kTile; ++j) {
+        const Src32 sj = ts[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair32_far<PSI>(D2[r], E[r], sj, F[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T[r][0] = fmaf(D[r][0], F[r][0], F[r][1]);
+        T[r][1] = fmaf(D[r][1], F[r][0], F[r][2]);
+        T[r][2] = fmaf(D[r][2], F[r][0], F[r][3]);
+        T[r][3] = F[r][4];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
+    if (more) sh[buf ^ 1][threadIdx.x] = nxt;
+    __syncthreads();
   }
   store_partials<R>(a, idx, S);
 }
@@ -485,6 +672,8 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 // Launch shape of one evaluation.
 struct Shape {
+  float near_factor = kNearDefault;
+  bool lds = false;  // FP32C: stage source tiles through LDS
   int R = 2;
   int splits = 1;
   int ntiles = 0;
@@ -510,8 +699,9 @@ Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
 // Device buffers of one evaluation: sources, partials, FP32C extras.
 struct EvalBuffers {
   double4* src64 = nullptr;
-  float4* src32 = nullptr;
+  Src32* src32 = nullptr;
   double4* ctr = nullptr;
+  float* trad = nullptr;
   double* srcq = nullptr;
   double* qtot = nullptr;
   double* part = nullptr;
@@ -541,10 +731,16 @@ struct EvalRequest {
 template <int R>
 void launch_pair(const EvalRequest& q, const EvalBuffers& b, const PairArgs& pa, dim3 grid, hipStream_t st) {
   if (q.prec == LPM_PREC_FP32C) {
-    if (q.want_psi)
-      hipLaunchKernelGGL((k_pair_f32c<R, true>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, pa);
-    else
-      hipLaunchKernelGGL((k_pair_f32c<R, false>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, pa);
+    if (q.shape.lds) {
+      if (q.want_psi)
+        hipLaunchKernelGGL((k_pair_f32c_lds<R, true>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa);
+      else
+        hipLaunchKernelGGL((k_pair_f32c_lds<R, false>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa);
+    } else if (q.want_psi) {
+      hipLaunchKernelGGL((k_pair_f32c<R, true>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa);
+    } else {
+      hipLaunchKernelGGL((k_pair_f32c<R, false>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa);
+    }
   } else if (q.form == FORM_DOT) {
     if (q.want_psi)
       hipLaunchKernelGGL((k_pair_f64<R, FORM_DOT, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
@@ -564,7 +760,7 @@ int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
   const Shape& sh = q.shape;
   const int ns_pad = sh.ntiles * kTile;
   if (q.prec == LPM_PREC_FP32C) {
-    hipLaunchKernelGGL(k_prep_f32c, dim3(sh.ntiles), dim3(kTile), 0, st, b.src32, b.ctr, b.srcq, q.srcpos, q.zeta,
+    hipLaunchKernelGGL(k_prep_f32c, dim3(sh.ntiles), dim3(kTile), 0, st, b.src32, b.ctr, b.trad, b.srcq, q.srcpos, q.zeta,
                        q.area, q.smap, q.ns);
     if (q.want_psi) hipLaunchKernelGGL(k_qsum, dim3(1), dim3(kBlock), 0, st, b.ctr, sh.ntiles, b.qtot);
   } else if (q.form == FORM_DOT) {
@@ -583,6 +779,7 @@ int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
   pa.tiles_per_split = sh.tiles_per_split;
   pa.part = b.part;
   pa.nt_pad = q.nt;
+  pa.near_factor = sh.near_factor;
   const dim3 grid(cdiv(q.nt, (long)kBlock * sh.R), sh.splits);
   if (ev0) LPM_HIP(hipEventRecord(ev0, st));
   switch (sh.R) {
@@ -617,14 +814,15 @@ int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
 
 // Sizes (bytes) of the evaluation buffers for ns sources, nt targets.
 struct BufferPlan {
-  size_t src64, src32, ctr, srcq, part;
+  size_t src64, src32, ctr, trad, srcq, part;
 };
 BufferPlan plan_buffers(int nt, const Shape& sh) {
   const size_t ns_pad = (size_t)sh.ntiles * kTile;
   BufferPlan p;
   p.src64 = ns_pad * sizeof(double4);
-  p.src32 = ns_pad * sizeof(float4);
+  p.src32 = ns_pad * sizeof(Src32);
   p.ctr = (size_t)sh.ntiles * sizeof(double4) + 256;
+  p.trad = (size_t)sh.ntiles * sizeof(float) + 256;
   p.srcq = ns_pad * sizeof(double);
   p.part = (size_t)sh.splits * kNC * nt * sizeof(double);
   return p;
@@ -755,6 +953,8 @@ struct lpm_bve_ctx {
   std::vector<void*> allocations;
   // launch shape
   int R_req = 2, splits_req = 0;
+  float near_factor = lpm::dev::kNearDefault;
+  bool lds = true;
   // velocity validity: form and precision that produced u
   bool vel_valid = false;
   int vel_prec = -1, vel_form = -1;
@@ -766,8 +966,11 @@ struct lpm_bve_ctx {
   struct GraphKey {
     double dt, omega;
     int scheme, prec, R, splits;
+    float near_factor;
+    bool lds;
     bool operator<(const GraphKey& o) const {
-      return std::tie(dt, omega, scheme, prec, R, splits) < std::tie(o.dt, o.omega, o.scheme, o.prec, o.R, o.splits);
+      return std::tie(dt, omega, scheme, prec, R, splits, near_factor, lds) <
+             std::tie(o.dt, o.omega, o.scheme, o.prec, o.R, o.splits, o.near_factor, o.lds);
     }
   };
   std::map<GraphKey, hipGraphExec_t> graphs;
@@ -791,7 +994,12 @@ void free_ctx_memory(lpm_bve_ctx* c) {
   c->timing_events.clear();
 }
 
-Shape ctx_shape(const lpm_bve_ctx* c) { return choose_shape(c->nt, std::max(c->ns, 1), c->R_req, c->splits_req); }
+Shape ctx_shape(const lpm_bve_ctx* c) {
+  Shape sh = choose_shape(c->nt, std::max(c->ns, 1), c->R_req, c->splits_req);
+  sh.near_factor = c->near_factor;
+  sh.lds = c->lds;
+  return sh;
+}
 
 // Make sure the evaluation buffers fit the current launch shape.
 int ensure_eval_buffers(lpm_bve_ctx* c, const Shape& sh) {
@@ -821,6 +1029,7 @@ int ensure_eval_buffers(lpm_bve_ctx* c, const Shape& sh) {
   if ((rc = alloc((void**)&c->eb.src64, bp.src64))) return rc;
   if ((rc = alloc((void**)&c->eb.src32, bp.src32))) return rc;
   if ((rc = alloc((void**)&c->eb.ctr, bp.ctr))) return rc;
+  if ((rc = alloc((void**)&c->eb.trad, bp.trad))) return rc;
   if ((rc = alloc((void**)&c->eb.srcq, bp.srcq))) return rc;
   if ((rc = alloc((void**)&c->eb.qtot, 256))) return rc;
   if ((rc = alloc((void**)&c->eb.part, bp.part))) return rc;
@@ -1099,7 +1308,7 @@ int lpm_bve_advance(lpm_bve_ctx* c, double dt, double omega, int scheme, int pre
   const Shape sh = ctx_shape(c);
   if ((rc = ensure_eval_buffers(c, sh))) return rc;
   if (use_graph && !c->timing && nsteps > 0) {
-    lpm_bve_ctx::GraphKey key{dt, omega, scheme, prec, sh.R, sh.splits};
+    lpm_bve_ctx::GraphKey key{dt, omega, scheme, prec, sh.R, sh.splits, sh.near_factor, sh.lds};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       hipGraph_t g;
@@ -1195,6 +1404,19 @@ int lpm_bve_set_launch(lpm_bve_ctx* c, int targets_per_lane, int source_splits) 
   if (source_splits < 0) return fail(LPM_ERR_ARG, "lpm_bve_set_launch: source_splits < 0");
   c->R_req = targets_per_lane;
   c->splits_req = source_splits;
+  return ok();
+}
+
+int lpm_bve_set_source_staging(lpm_bve_ctx* c, int lds) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_source_staging: null context");
+  c->lds = lds != 0;
+  return ok();
+}
+
+int lpm_bve_set_near_factor(lpm_bve_ctx* c, double factor) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_near_factor: null context");
+  if (!(factor >= 0.0)) return fail(LPM_ERR_ARG, "lpm_bve_set_near_factor: factor must be >= 0");
+  c->near_factor = (float)std::min(factor, 1e18);
   return ok();
 }
 

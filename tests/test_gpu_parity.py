@@ -263,3 +263,23 @@ def test_circulation(omesh):
     c.advance(0.01, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=5)
     assert abs(c.circulation() - g0) < 1e-6 * float(np.sum(np.abs(fz) * lm.farea))
     c.close()
+
+
+@pytest.mark.parametrize("lds,near", [(False, 4.0), (True, 4.0), (True, 2.0), (True, 1e30), (False, 1e30)])
+def test_fp32c_variants_agree(omesh, lds, near):
+    """Source staging (scalar cache or LDS) and the near/far threshold change
+    only rounding: every variant stays within the FP32C bound of FP64."""
+    m = omesh(5)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=2, nvort=16, sigma=0.1)
+    fz = fz[m.fmask == 0]
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
+    a = c.get_state()
+    c.set_source_staging(lds)
+    c.set_near_factor(near)
+    c.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
+    b = c.get_state()
+    assert max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"])) < 1e-6
+    assert max(normwise(b["vpsi"], a["vpsi"]), normwise(b["fpsi"], a["fpsi"])) < 1e-6
+    c.close()
