@@ -72,7 +72,7 @@ def cpu_baseline(mesh, fz_tree, budget_s: float):
         O.velocity(mesh.fx, mesh.fx, fz_tree, mesh.farea, mesh.fmask, True, stride=stride)
         return time.perf_counter() - t0, len(range(0, nt, stride))
 
-    dt, n = run(max(1, nt // (4 * threads)))  # calibration
+    dt, n = run(max(1, nt // (64 * threads)))  # calibration: 64 targets per thread
     per_target = dt / n
     want = max(threads, int(budget_s / per_target))
     stride = max(1, nt // want)

@@ -168,8 +168,8 @@ int lpm_bve_set_launch(lpm_bve_ctx* ctx, int targets_per_lane, int source_splits
  * source always uses the explicit form. */
 int lpm_bve_set_near_factor(lpm_bve_ctx* ctx, double factor);
 
-/* FP32C: stage source tiles through LDS (1, default) or read them through
- * the scalar cache (0). */
+/* Stage source tiles through LDS (1, default) or read them through the
+ * scalar cache (0); both precisions. */
 int lpm_bve_set_source_staging(lpm_bve_ctx* ctx, int lds);
 
 /* The context's HIP stream (hipStream_t) and device id. */

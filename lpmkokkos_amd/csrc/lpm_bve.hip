@@ -263,6 +263,46 @@ __global__ __launch_bounds__(kBlock) void k_pair_f64(const double4* __restrict__
   store_partials<R>(a, idx, S);
 }
 
+// FP64 pair sum with the source tiles staged through LDS (double-buffered,
+// coalesced 8 KB copies per tile), as k_pair_f32c_lds.
+template <int R, int FORM, bool PSI>
+__global__ __launch_bounds__(kBlock) void k_pair_f64_lds(const double4* __restrict__ src, PairArgs a) {
+  __shared__ double4 sh[2][kTile];
+  double x[R][3];
+  int self[R], idx[R];
+  load_targets<R>(a, x, self, idx);
+  double S[R][kNC];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
+  const int t0 = blockIdx.y * a.tiles_per_split;
+  const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
+  if (t0 < t1) sh[0][threadIdx.x] = src[(long)t0 * kTile + threadIdx.x];
+  __syncthreads();
+  for (int t = t0; t < t1; ++t) {
+    const int buf = (t - t0) & 1;
+    if (t + 1 < t1) sh[buf ^ 1][threadIdx.x] = src[(long)(t + 1) * kTile + threadIdx.x];
+    const int base = t * kTile;
+    const double4* ts = sh[buf];
+    if (!wave_has_self<R>(self, base)) {
+#pragma unroll 4
+      for (int j = 0; j < kTile; ++j) {
+        const double4 sj = ts[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair64<FORM, PSI, false>(x[r], sj, false, S[r]);
+      }
+    } else {
+      for (int j = 0; j < kTile; ++j) {
+        const double4 sj = ts[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pair64<FORM, PSI, true>(x[r], sj, base + j == self[r], S[r]);
+      }
+    }
+    __syncthreads();
+  }
+  store_partials<R>(a, idx, S);
+}
+
 // Near form: d = D + dx formed explicitly, f = q / |d|^2, S += f d.
 // P accumulates q log2(|d|^2).  `me` drops the target's own source.
 template <bool PSI, bool CHECK>
@@ -392,8 +432,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
 }
 
 // Same arithmetic as k_pair_f32c with the source tiles staged through LDS:
-// the workgroup copies tile t+1 (coalesced, 8 KB) while it works on tile t,
-// and waves read records with uniform-address LDS loads into VGPRs.
+// the workgroup copies tile t+1 (coalesced, 8 KB) into the second buffer as
+// it starts tile t, and waves read records with uniform-address LDS loads
+// into VGPRs (LDS returns in order, so the compiler prefetches the next
+// records while the current ones are in use).
 template <int R, bool PSI>
 __global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restrict__ src,
                                                           const double4* __restrict__ ctr,
@@ -412,9 +454,9 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restric
   __syncthreads();
   for (int t = t0; t < t1; ++t) {
     const int buf = (t - t0) & 1;
-    const bool more = t + 1 < t1;
-    Src32 nxt;
-    if (more) nxt = src[(long)(t + 1) * kTile + threadIdx.x];
+    // Stage tile t+1 into the other buffer.  Its last readers (tile t-1)
+    // finished before the barrier that ended the previous iteration.
+    if (t + 1 < t1) sh[buf ^ 1][threadIdx.x] = src[(long)(t + 1) * kTile + threadIdx.x];
     const int base = t * kTile;
     const double4 c = ctr[t];
     const float rn = a.near_factor * trad[t];
@@ -473,7 +515,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restric
 #pragma unroll
     for (int r = 0; r < R; ++r)
       for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
-    if (more) sh[buf ^ 1][threadIdx.x] = nxt;
     __syncthreads();
   }
   store_partials<R>(a, idx, S);
@@ -673,7 +714,7 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 // Launch shape of one evaluation.
 struct Shape {
   float near_factor = kNearDefault;
-  bool lds = false;  // FP32C: stage source tiles through LDS
+  bool lds = true;  // stage source tiles through LDS (else scalar-cache reads)
   int R = 2;
   int splits = 1;
   int ntiles = 0;
@@ -740,6 +781,18 @@ void launch_pair(const EvalRequest& q, const EvalBuffers& b, const PairArgs& pa,
       hipLaunchKernelGGL((k_pair_f32c<R, true>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa);
     } else {
       hipLaunchKernelGGL((k_pair_f32c<R, false>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa);
+    }
+  } else if (q.shape.lds) {
+    if (q.form == FORM_DOT) {
+      if (q.want_psi)
+        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_DOT, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
+      else
+        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_DOT, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
+    } else {
+      if (q.want_psi)
+        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_CHORD, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
+      else
+        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_CHORD, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
     }
   } else if (q.form == FORM_DOT) {
     if (q.want_psi)

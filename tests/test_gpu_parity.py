@@ -283,3 +283,23 @@ def test_fp32c_variants_agree(omesh, lds, near):
     assert max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"])) < 1e-6
     assert max(normwise(b["vpsi"], a["vpsi"]), normwise(b["fpsi"], a["fpsi"])) < 1e-6
     c.close()
+
+
+@pytest.mark.parametrize("lds", [True, False])
+def test_fp64_staging_variants(omesh, lds):
+    """FP64 pair sums read through LDS or the scalar cache agree with the oracle."""
+    m = omesh(4)
+    lm = LeafMesh(m)
+    vz, fz = rh_field(lm.vx), rh_field(lm.fx)
+    c = make_ctx(lm, vz, fz)
+    c.set_source_staging(lds)
+    for scheme, fn in ((L.SCHEME_REF, O.velocity), (L.SCHEME_LAGR, O.velocity_chord)):
+        c.solve(L.PREC_FP64, scheme, want_psi=True)
+        s = c.get_state()
+        ov = O.velocity(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False) if scheme == L.SCHEME_REF else \
+            O.velocity_chord(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False)
+        of = fn(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True)
+        assert normwise(s["vu"], ov) < 1e-13 and normwise(s["fu"], of) < 1e-13
+    ps = O.streamfn_chord(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True)
+    assert normwise(s["fpsi"], ps) < 1e-13
+    c.close()
