@@ -89,6 +89,100 @@ def cpu_baseline(mesh, fz_tree, budget_s: float):
     }
 
 
+def serving_form(nreq: int, max_level: int, load: float = 0.8, steps: int = 20, seed: int = 4):
+    """Continuous-batching sample of the serving configuration: Poisson
+    arrivals at `load` x measured capacity, levels from {4..max_level} with the
+    configuration's probabilities (renormalised), RH n in 2..6 + 4..16
+    Gaussian vortices, 20 FP32C steps each, 200 GB admission budget."""
+    from lpmkokkos_amd import serve as S
+    probs = {l: p for l, p in S.LEVEL_PROBS.items() if l <= max_level}
+    tot = sum(probs.values())
+    probs = {l: p / tot for l, p in probs.items()}
+    svc = S.measure_service_times(sorted(probs), steps=steps)
+    mean_service = sum(probs[l] * svc[l] for l in probs)
+    rate = load / mean_service
+    reqs = S.make_requests(nreq, rate, seed=seed, level_probs=probs, steps=steps)
+    srv = S.Server(budget_bytes=200e9)
+    res = srv.run(reqs)
+    srv.close()
+    res.update({"arrival_rate_per_s": rate, "load": load, "level_probs": probs,
+                "service_s_by_level": {int(k): v for k, v in svc.items()}, "steps_per_request": steps,
+                "sample": f"{nreq} requests (levels <= {max_level}) of the 2000-request serving configuration"})
+    return res
+
+
+def ensemble_form(args) -> int:
+    """Batching configuration: 64 members, levels uniform on 3..7, RH wave
+    (K x U(0.95, 1.05)) + 8 Gaussian vortices, 10 FP32C steps, one persistent
+    pair kernel per evaluation, step graph replayed; members checked against
+    the same problems run alone."""
+    import lpmkokkos_amd as L
+    from lpmkokkos_amd import ensemble as E
+    specs = E.ensemble_config(seed=3, n=args.ensemble_members)
+    t0 = time.perf_counter()
+    members = [E.make_member(s) for s in specs]
+    log(f"[bench] ensemble: {len(members)} members built in {time.perf_counter() - t0:.1f} s")
+    tuned = {}
+    for s, c in zip(specs, members):
+        if s.level not in tuned:
+            tuned[s.level] = E.autotune(c, reps=1)
+    log(f"[bench] autotuned shapes (R, splits, ms): {tuned}")
+    ens = E.Ensemble(0)
+    for c in members:
+        ens.add(c)
+    ens.advance(DT, OMEGA, nsteps=1)  # warm: entry velocities, plan, capture
+    ens.synchronize()
+    nsteps = 10  # the batching configuration's step count
+    t1 = time.perf_counter()
+    ens.advance(DT, OMEGA, nsteps=nsteps)
+    ens.synchronize()
+    dt_ens = time.perf_counter() - t1
+    pairs_step = sum(4.0 * ((2 + 10 * 4 ** s.level + 20 * 4 ** s.level) * 20 * 4 ** s.level - 20 * 4 ** s.level)
+                     for s in specs)
+    # the same steps run member by member (each alone, own graph)
+    alone = [E.make_member(s) for s in specs[:8]]
+    for c in alone:
+        c.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1 + nsteps)
+    same = all(np.array_equal(a.get_state()[k], b.get_state()[k])
+               for a, b in zip(alone, members[:8]) for k in ("vx", "fx", "vu", "fu", "fzeta"))
+    t2 = time.perf_counter()
+    seq = [E.make_member(s) for s in specs]
+    for c in seq:
+        c.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1)
+    for c in seq:
+        c.synchronize()
+    t3 = time.perf_counter()
+    for c in seq:  # one problem at a time, as a caller without batching would run them
+        c.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=nsteps)
+        c.synchronize()
+    dt_seq = time.perf_counter() - t3
+    out = {
+        "metric": "ensemble pair-interactions/s (batching form)",
+        "value": pairs_step * nsteps / dt_ens,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "ms_per_step": 1e3 * dt_ens / nsteps,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(specs)} members, levels uniform on 3..7 (seed 3), RH n=4 K x U(0.95,1.05) "
+                               "+ 8 Gaussian vortices N(0,0.05) sigma 0.15, Lagrangian RK4 dt=0.005",
+                   "levels": {int(l): sum(1 for s in specs if s.level == l) for l in sorted({s.level for s in specs})},
+                   "pairs_per_step": pairs_step},
+        "members_sequential_pairs_per_s": pairs_step * nsteps / dt_seq,
+        "ensemble_speedup_vs_sequential": dt_seq / dt_ens,
+        "first8_members_bitwise_equal_alone": bool(same),
+        "autotuned_shapes": {int(k): v for k, v in tuned.items()},
+        "build_s": t2 - t0,
+    }
+    print(json.dumps(out), flush=True)
+    ens.close()
+    for c in members + alone + seq:
+        c.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +197,10 @@ def main() -> int:
     ap.add_argument("--splits", type=int, default=0)
     ap.add_argument("--near-factor", type=float, default=None)
     ap.add_argument("--staging", choices=["lds", "scalar"], default=None)
+    ap.add_argument("--form", choices=["precision", "ensemble", "serving"], default="precision")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-max-level", type=int, default=7)
+    ap.add_argument("--ensemble-members", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the multi-GPU sharding is a design "
@@ -110,6 +208,13 @@ def main() -> int:
         return 2
 
     import lpmkokkos_amd as L
+
+    if args.form == "ensemble":
+        return ensemble_form(args)
+    if args.form == "serving":
+        out = serving_form(args.serve_requests, args.serve_max_level)
+        print(json.dumps(out), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     mesh, fx, farea, vz, fz = build_problem(args.level)
@@ -190,6 +295,12 @@ def main() -> int:
         cpu = cpu_baseline(mesh, mesh_z, args.cpu_seconds)
         log(f"[bench] CPU baseline: {cpu['value']:.4e} pairs/s on {cpu['cores']} threads")
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serving_form(args.serve_requests, args.serve_max_level)
+        log(f"[bench] serving sample: p50 {serving['latency_p50_s']:.2f} s, p99 {serving['latency_p99_s']:.2f} s, "
+            f"{serving['pairs_per_s']:.4e} pairs/s")
+
     out = {
         "metric": "Biot-Savart pair-interactions/s, FP32c vs FP64 at 1.31M src; p50/p99 latency",
         "value": rate,
@@ -226,6 +337,9 @@ def main() -> int:
             "pairs_per_launch": pairs_eval,
         },
         "cpu_baseline": cpu,
+        "serving": serving,
+        "latency_p50_s": serving["latency_p50_s"] if serving else None,
+        "latency_p99_s": serving["latency_p99_s"] if serving else None,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -87,6 +87,10 @@ int lpm_ic_gauss_params(uint64_t seed, int nvort, double amp_scale, double* cent
 int lpm_ic_eval(const double* xyz, int64_t n, double* zeta, double sbr, int rh_n, double rh_w, double rh_K,
                 int nvort, const double* centers, const double* amps, double sigma);
 
+/* The same generator's raw draws: nuniform values in [0,1), then nnormal
+ * standard normals (used to draw ensemble and request parameters). */
+int lpm_rng_draws(uint64_t seed, int64_t nuniform, double* uniform, int64_t nnormal, double* normal);
+
 /* Area-weighted mean of zeta over unmasked faces, summed in index order. */
 int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, double* mean);
 
@@ -157,8 +161,8 @@ int lpm_bve_synchronize(lpm_bve_ctx* ctx);
 /* Sizes of the resident problem: {nv, nf, nsources, ntargets}. */
 int lpm_bve_sizes(lpm_bve_ctx* ctx, int64_t* sizes4);
 
-/* Kernel launch shape: targets per lane (1, 2 or 4) and source splits
- * (0 = automatic).  Applies to later solves and steps. */
+/* Kernel launch shape: targets per lane (1, 2 or 4; 0 = automatic) and
+ * source splits (0 = automatic).  Applies to later solves and steps. */
 int lpm_bve_set_launch(lpm_bve_ctx* ctx, int targets_per_lane, int source_splits);
 
 /* FP32C: source tiles whose centre lies within `factor` tile radii of a
@@ -182,6 +186,48 @@ int lpm_bve_last_pair_timing(lpm_bve_ctx* ctx, int* launches, double* mean_ms);
 /* Record HIP events around every pair-sum kernel this context queues (1) or
  * stop (0).  While on, steps run without graph replay. */
 int lpm_bve_enable_timing(lpm_bve_ctx* ctx, int on);
+
+/* Device memory a context needs for nv vertices and nactive active faces. */
+int lpm_bve_state_bytes(int64_t nv, int64_t nactive, int64_t* bytes);
+
+/* ------------------------------------------------------------------ */
+/* Launch shapes keyed on the number of targets N (vertices + active   */
+/* faces).  A context whose launch is automatic (lpm_bve_set_launch    */
+/* with targets_per_lane 0, the default) uses the rule for its N, so a */
+/* problem alone and inside an ensemble is decomposed identically.     */
+/* ------------------------------------------------------------------ */
+int lpm_shape_rule_set(int64_t ntargets, int targets_per_lane, int source_splits);
+int lpm_shape_rule_get(int64_t ntargets, int* targets_per_lane, int* source_splits);
+int lpm_shape_rules_clear(void);
+
+/* Time one evaluation per candidate shape on the context's problem and
+ * record the fastest as the rule for its N.  Outputs may be NULL. */
+int lpm_bve_autotune(lpm_bve_ctx* ctx, int prec, int scheme, int reps, int* best_targets_per_lane,
+                     int* best_source_splits, double* best_ms);
+
+/* ------------------------------------------------------------------ */
+/* Ensemble: independent problems (contexts, possibly of different     */
+/* mesh levels) advanced together.  Each of the four evaluations of a  */
+/* step runs as ONE persistent pair kernel over the flattened list of  */
+/* (member, target block, source split) items, longest first; each     */
+/* member keeps the decomposition it has alone and writes its own      */
+/* partial slots, so its results are bit-identical alone and in the    */
+/* ensemble.  Members are borrowed, must stay alive while they belong  */
+/* to the ensemble, and must not be stepped on their own concurrently. */
+/* ------------------------------------------------------------------ */
+typedef struct lpm_bve_ensemble lpm_bve_ensemble;
+
+int lpm_ens_create(int device, lpm_bve_ensemble** ens);
+int lpm_ens_destroy(lpm_bve_ensemble* ens);
+int lpm_ens_add(lpm_bve_ensemble* ens, lpm_bve_ctx* member);
+int lpm_ens_remove(lpm_bve_ensemble* ens, lpm_bve_ctx* member);
+int lpm_ens_size(lpm_bve_ensemble* ens, int* nmembers);
+/* Workgroups per CU of the persistent kernel (0 = occupancy limit). */
+int lpm_ens_set_blocks_per_cu(lpm_bve_ensemble* ens, int blocks_per_cu);
+/* `nsteps` RK4 steps of every member (same dt, omega, scheme, precision). */
+int lpm_ens_advance(lpm_bve_ensemble* ens, double dt, double omega, int scheme, int prec, int nsteps,
+                    int use_graph);
+int lpm_ens_synchronize(lpm_bve_ensemble* ens);
 
 #ifdef __cplusplus
 }

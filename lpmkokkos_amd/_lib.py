@@ -36,6 +36,7 @@ SIGNATURES = {
     "lpm_ic_eval": [_dp, C.c_int64, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dpn, _dpn,
                     C.c_double],
     "lpm_area_mean": [_dp, _dp, _bp, C.c_int64, C.POINTER(C.c_double)],
+    "lpm_rng_draws": [C.c_uint64, C.c_int64, _dpn, C.c_int64, _dpn],
     "lpm_bve_vertex_velocity": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int64, C.c_void_p],
     "lpm_bve_face_velocity": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
@@ -58,6 +59,20 @@ SIGNATURES = {
     "lpm_bve_stream": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int)],
     "lpm_bve_last_pair_timing": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "lpm_bve_enable_timing": [C.c_void_p, C.c_int],
+    "lpm_bve_state_bytes": [C.c_int64, C.c_int64, C.POINTER(C.c_int64)],
+    "lpm_shape_rule_set": [C.c_int64, C.c_int, C.c_int],
+    "lpm_shape_rule_get": [C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "lpm_shape_rules_clear": [],
+    "lpm_bve_autotune": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                         C.POINTER(C.c_double)],
+    "lpm_ens_create": [C.c_int, C.POINTER(C.c_void_p)],
+    "lpm_ens_destroy": [C.c_void_p],
+    "lpm_ens_add": [C.c_void_p, C.c_void_p],
+    "lpm_ens_remove": [C.c_void_p, C.c_void_p],
+    "lpm_ens_size": [C.c_void_p, C.POINTER(C.c_int)],
+    "lpm_ens_set_blocks_per_cu": [C.c_void_p, C.c_int],
+    "lpm_ens_advance": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
+    "lpm_ens_synchronize": [C.c_void_p],
 }
 
 _lib = None

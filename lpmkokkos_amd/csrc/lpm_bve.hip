@@ -167,10 +167,11 @@ struct PairArgs {
 };
 
 template <int R>
-__device__ __forceinline__ void load_targets(const PairArgs& a, double (&x)[R][3], int (&self)[R], int (&idx)[R]) {
+__device__ __forceinline__ void load_targets(const PairArgs& a, int bx, double (&x)[R][3], int (&self)[R],
+                                             int (&idx)[R]) {
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int i = (blockIdx.x * R + r) * kBlock + threadIdx.x;
+    const int i = (bx * R + r) * kBlock + threadIdx.x;
     idx[r] = i;
     const int ii = min(i, a.nt - 1);
     x[r][0] = a.tgt.at(ii, 0);
@@ -189,8 +190,9 @@ __device__ __forceinline__ bool wave_has_self(const int (&self)[R], int base) {
 }
 
 template <int R>
-__device__ __forceinline__ void store_partials(const PairArgs& a, const int (&idx)[R], const double (&S)[R][kNC]) {
-  double* out = a.part + (long)blockIdx.y * kNC * a.nt_pad;
+__device__ __forceinline__ void store_partials(const PairArgs& a, int by, const int (&idx)[R],
+                                               const double (&S)[R][kNC]) {
+  double* out = a.part + (long)by * kNC * a.nt_pad;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (idx[r] < a.nt) {
@@ -235,7 +237,7 @@ template <int R, int FORM, bool PSI>
 __global__ __launch_bounds__(kBlock) void k_pair_f64(const double4* __restrict__ src, PairArgs a) {
   double x[R][3];
   int self[R], idx[R];
-  load_targets<R>(a, x, self, idx);
+  load_targets<R>(a, blockIdx.x, x, self, idx);
   double S[R][kNC];
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -260,22 +262,22 @@ __global__ __launch_bounds__(kBlock) void k_pair_f64(const double4* __restrict__
       }
     }
   }
-  store_partials<R>(a, idx, S);
+  store_partials<R>(a, blockIdx.y, idx, S);
 }
 
 // FP64 pair sum with the source tiles staged through LDS (double-buffered,
 // coalesced 8 KB copies per tile), as k_pair_f32c_lds.
 template <int R, int FORM, bool PSI>
-__global__ __launch_bounds__(kBlock) void k_pair_f64_lds(const double4* __restrict__ src, PairArgs a) {
-  __shared__ double4 sh[2][kTile];
+__device__ __forceinline__ void pair_block_f64_lds(const double4* __restrict__ src, const PairArgs& a, int bx,
+                                                   int by, double4 (*sh)[kTile]) {
   double x[R][3];
   int self[R], idx[R];
-  load_targets<R>(a, x, self, idx);
+  load_targets<R>(a, bx, x, self, idx);
   double S[R][kNC];
 #pragma unroll
   for (int r = 0; r < R; ++r)
     for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
-  const int t0 = blockIdx.y * a.tiles_per_split;
+  const int t0 = by * a.tiles_per_split;
   const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
   if (t0 < t1) sh[0][threadIdx.x] = src[(long)t0 * kTile + threadIdx.x];
   __syncthreads();
@@ -300,7 +302,13 @@ __global__ __launch_bounds__(kBlock) void k_pair_f64_lds(const double4* __restri
     }
     __syncthreads();
   }
-  store_partials<R>(a, idx, S);
+  store_partials<R>(a, by, idx, S);
+}
+
+template <int R, int FORM, bool PSI>
+__global__ __launch_bounds__(kBlock) void k_pair_f64_lds(const double4* __restrict__ src, PairArgs a) {
+  __shared__ double4 sh[2][kTile];
+  pair_block_f64_lds<R, FORM, PSI>(src, a, blockIdx.x, blockIdx.y, sh);
 }
 
 // Near form: d = D + dx formed explicitly, f = q / |d|^2, S += f d.
@@ -350,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
                                                       const float* __restrict__ trad, PairArgs a) {
   double x[R][3];
   int self[R], idx[R];
-  load_targets<R>(a, x, self, idx);
+  load_targets<R>(a, blockIdx.x, x, self, idx);
   double S[R][kNC];
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -428,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
     for (int r = 0; r < R; ++r)
       for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
   }
-  store_partials<R>(a, idx, S);
+  store_partials<R>(a, blockIdx.y, idx, S);
 }
 
 // Same arithmetic as k_pair_f32c with the source tiles staged through LDS:
@@ -437,18 +445,17 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
 // into VGPRs (LDS returns in order, so the compiler prefetches the next
 // records while the current ones are in use).
 template <int R, bool PSI>
-__global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restrict__ src,
-                                                          const double4* __restrict__ ctr,
-                                                          const float* __restrict__ trad, PairArgs a) {
-  __shared__ Src32 sh[2][kTile];
+__device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ src, const double4* __restrict__ ctr,
+                                                    const float* __restrict__ trad, const PairArgs& a, int bx,
+                                                    int by, Src32 (*sh)[kTile]) {
   double x[R][3];
   int self[R], idx[R];
-  load_targets<R>(a, x, self, idx);
+  load_targets<R>(a, bx, x, self, idx);
   double S[R][kNC];
 #pragma unroll
   for (int r = 0; r < R; ++r)
     for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
-  const int t0 = blockIdx.y * a.tiles_per_split;
+  const int t0 = by * a.tiles_per_split;
   const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
   if (t0 < t1) sh[0][threadIdx.x] = src[(long)t0 * kTile + threadIdx.x];
   __syncthreads();
@@ -517,7 +524,78 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restric
       for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
     __syncthreads();
   }
-  store_partials<R>(a, idx, S);
+  store_partials<R>(a, by, idx, S);
+}
+
+template <int R, bool PSI>
+__global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restrict__ src,
+                                                          const double4* __restrict__ ctr,
+                                                          const float* __restrict__ trad, PairArgs a) {
+  __shared__ Src32 sh[2][kTile];
+  pair_block_f32c_lds<R, PSI>(src, ctr, trad, a, blockIdx.x, blockIdx.y, sh);
+}
+
+// ---------------------------------------------------------------------------
+// ensemble: one persistent launch over a flattened work list
+// ---------------------------------------------------------------------------
+
+// Per-member arguments of a pair evaluation.
+struct EnsMember {
+  PairArgs a;
+  const Src32* src32;
+  const double4* ctr;
+  const float* trad;
+  const double4* src64;
+};
+
+// One work item: target block bx and source split by of member m.  Items
+// are processed in list order by whichever workgroup takes them next; each
+// item writes its own partial slots, so the result does not depend on which
+// workgroup ran it or on the other members in the list.
+struct EnsItem {
+  int member, bx, by;
+};
+
+__global__ void k_reset_counter(int* counter) {
+  if (threadIdx.x == 0) *counter = 0;
+}
+
+__device__ __forceinline__ int next_item(int* counter, int* slot) {
+  if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
+  __syncthreads();
+  const int it = *slot;
+  __syncthreads();
+  return it;
+}
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_pair_f32c_ens(const EnsMember* __restrict__ mem,
+                                                          const EnsItem* __restrict__ items, int nitems,
+                                                          int* counter) {
+  __shared__ Src32 sh[2][kTile];
+  __shared__ int slot;
+  for (;;) {
+    const int it = next_item(counter, &slot);
+    if (it >= nitems) break;
+    const EnsItem w = items[it];
+    const EnsMember& m = mem[w.member];
+    pair_block_f32c_lds<R, false>(m.src32, m.ctr, m.trad, m.a, w.bx, w.by, sh);
+  }
+}
+
+template <int R, int FORM>
+__global__ __launch_bounds__(kBlock) void k_pair_f64_ens(const EnsMember* __restrict__ mem,
+                                                         const EnsItem* __restrict__ items, int nitems,
+                                                         int* counter) {
+  __shared__ double4 sh[2][kTile];
+  __shared__ int slot;
+  for (;;) {
+    const int it = next_item(counter, &slot);
+    if (it >= nitems) break;
+    const EnsItem w = items[it];
+    const EnsMember& m = mem[w.member];
+    pair_block_f64_lds<R, FORM, false>(m.src64, m.a, w.bx, w.by, sh);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -721,7 +799,25 @@ struct Shape {
   int tiles_per_split = 0;
 };
 
+// Launch shapes chosen by the autotuner, keyed on the number of targets N.
+// A context whose launch is automatic takes the entry for its N, so a
+// problem run alone and the same problem inside an ensemble use the same
+// decomposition (and get the same bits).
+struct ShapeRule {
+  int R, splits;
+};
+std::mutex g_rule_mu;
+std::map<int, ShapeRule> g_rules;
+
 Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
+  if (R_req == 0) {
+    std::lock_guard<std::mutex> g(g_rule_mu);
+    auto it = g_rules.find(nt);
+    if (it != g_rules.end()) {
+      R_req = it->second.R;
+      if (splits_req == 0) splits_req = it->second.splits;
+    }
+  }
   Shape sh;
   sh.R = (R_req == 1 || R_req == 2 || R_req == 4) ? R_req : 2;
   sh.ntiles = std::max(1, cdiv(ns, kTile));
@@ -807,14 +903,26 @@ void launch_pair(const EvalRequest& q, const EvalBuffers& b, const PairArgs& pa,
   }
 }
 
-// Queue prep + pair + finalize on `st`.  ev0/ev1 (optional) bracket the pair
-// kernel.
-int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+PairArgs make_pair_args(const EvalRequest& q, const EvalBuffers& b) {
+  PairArgs pa;
+  pa.tgt = q.tgt;
+  pa.self_slot = q.self_slot;
+  pa.nt = q.nt;
+  pa.ntiles = q.shape.ntiles;
+  pa.tiles_per_split = q.shape.tiles_per_split;
+  pa.part = b.part;
+  pa.nt_pad = q.nt;
+  pa.near_factor = q.shape.near_factor;
+  return pa;
+}
+
+// Source records for one evaluation.
+int enqueue_prep(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
   const Shape& sh = q.shape;
   const int ns_pad = sh.ntiles * kTile;
   if (q.prec == LPM_PREC_FP32C) {
-    hipLaunchKernelGGL(k_prep_f32c, dim3(sh.ntiles), dim3(kTile), 0, st, b.src32, b.ctr, b.trad, b.srcq, q.srcpos, q.zeta,
-                       q.area, q.smap, q.ns);
+    hipLaunchKernelGGL(k_prep_f32c, dim3(sh.ntiles), dim3(kTile), 0, st, b.src32, b.ctr, b.trad, b.srcq, q.srcpos,
+                       q.zeta, q.area, q.smap, q.ns);
     if (q.want_psi) hipLaunchKernelGGL(k_qsum, dim3(1), dim3(kBlock), 0, st, b.ctr, sh.ntiles, b.qtot);
   } else if (q.form == FORM_DOT) {
     hipLaunchKernelGGL((k_prep_f64<FORM_DOT>), dim3(cdiv(ns_pad, kBlock)), dim3(kBlock), 0, st, b.src64, q.srcpos,
@@ -824,27 +932,29 @@ int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
                        q.srcpos, q.zeta, q.area, q.smap, q.ns, ns_pad);
   }
   LPM_HIP(hipGetLastError());
-  PairArgs pa;
-  pa.tgt = q.tgt;
-  pa.self_slot = q.self_slot;
-  pa.nt = q.nt;
-  pa.ntiles = sh.ntiles;
-  pa.tiles_per_split = sh.tiles_per_split;
-  pa.part = b.part;
-  pa.nt_pad = q.nt;
-  pa.near_factor = sh.near_factor;
-  const dim3 grid(cdiv(q.nt, (long)kBlock * sh.R), sh.splits);
+  return LPM_OK;
+}
+
+// The pair kernel over the request's own grid; ev0/ev1 (optional) bracket it.
+int enqueue_pair(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  const PairArgs pa = make_pair_args(q, b);
+  const dim3 grid(cdiv(q.nt, (long)kBlock * q.shape.R), q.shape.splits);
   if (ev0) LPM_HIP(hipEventRecord(ev0, st));
-  switch (sh.R) {
+  switch (q.shape.R) {
     case 1: launch_pair<1>(q, b, pa, grid, st); break;
     case 4: launch_pair<4>(q, b, pa, grid, st); break;
     default: launch_pair<2>(q, b, pa, grid, st); break;
   }
   LPM_HIP(hipGetLastError());
   if (ev1) LPM_HIP(hipEventRecord(ev1, st));
+  return LPM_OK;
+}
+
+// Sum split partials, cross product, stream function.
+int enqueue_finalize(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
   FinArgs fa;
   fa.part = b.part;
-  fa.splits = sh.splits;
+  fa.splits = q.shape.splits;
   fa.nt = q.nt;
   fa.nt_pad = q.nt;
   fa.tgt = q.tgt;
@@ -863,6 +973,14 @@ int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
     hipLaunchKernelGGL((k_finalize<FIN_CHORD>), fg, dim3(kBlock), 0, st, fa);
   LPM_HIP(hipGetLastError());
   return LPM_OK;
+}
+
+// Queue prep + pair + finalize on `st`.
+int enqueue_eval(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  int rc;
+  if ((rc = enqueue_prep(q, b, st))) return rc;
+  if ((rc = enqueue_pair(q, b, st, ev0, ev1))) return rc;
+  return enqueue_finalize(q, b, st);
 }
 
 // Sizes (bytes) of the evaluation buffers for ns sources, nt targets.
@@ -1004,8 +1122,10 @@ struct lpm_bve_ctx {
   double* circ = nullptr;
   lpm::EvalBuffers eb;
   std::vector<void*> allocations;
-  // launch shape
-  int R_req = 2, splits_req = 0;
+  // launch shape (R_req 0: automatic -- autotuned rule for this N, else 2)
+  int R_req = 0, splits_req = 0;
+  // bumped whenever device buffers are (re)allocated
+  uint64_t generation = 0;
   float near_factor = lpm::dev::kNearDefault;
   bool lds = true;
   // velocity validity: form and precision that produced u
@@ -1071,6 +1191,7 @@ int ensure_eval_buffers(lpm_bve_ctx* c, const Shape& sh) {
     if (hipMalloc(&c->eb.part, bp.part) != hipSuccess) return fail(LPM_ERR_NOMEM, "partial buffer allocation failed");
     c->allocations.push_back(c->eb.part);
     c->eb.part_elems = part_elems;
+    ++c->generation;
     return LPM_OK;
   }
   auto alloc = [&](void** p, size_t n) -> int {
@@ -1087,15 +1208,17 @@ int ensure_eval_buffers(lpm_bve_ctx* c, const Shape& sh) {
   if ((rc = alloc((void**)&c->eb.qtot, 256))) return rc;
   if ((rc = alloc((void**)&c->eb.part, bp.part))) return rc;
   c->eb.part_elems = part_elems;
+  ++c->generation;
   return LPM_OK;
 }
 
-// Evaluate velocity (and psi) at positions (px, py, pz) with vorticity pz.
-int ctx_eval(lpm_bve_ctx* c, const double* px, const double* pzeta, int prec, int form, bool want_psi) {
+// Request for an evaluation at positions px ([3][nt] SoA) with vorticity
+// pzeta; makes sure the evaluation buffers fit the launch shape.
+int ctx_request(lpm_bve_ctx* c, const double* px, const double* pzeta, int prec, int form, bool want_psi,
+                EvalRequest& q) {
   const Shape sh = ctx_shape(c);
   int rc = ensure_eval_buffers(c, sh);
   if (rc) return rc;
-  EvalRequest q;
   q.tgt = Pos3{px, 1, (long)c->nt};
   q.nt = c->nt;
   q.srcpos = q.tgt;
@@ -1111,16 +1234,25 @@ int ctx_eval(lpm_bve_ctx* c, const double* px, const double* pzeta, int prec, in
   q.form = form;
   q.want_psi = want_psi;
   q.shape = sh;
+  return LPM_OK;
+}
+
+// Evaluate velocity (and psi) at positions px with vorticity pzeta on `st`.
+int ctx_eval(lpm_bve_ctx* c, const double* px, const double* pzeta, int prec, int form, bool want_psi,
+             hipStream_t st) {
+  EvalRequest q;
+  int rc = ctx_request(c, px, pzeta, prec, form, want_psi, q);
+  if (rc) return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     LPM_HIP(hipEventCreate(&e0));
     LPM_HIP(hipEventCreate(&e1));
     c->timing_events.push_back({e0, e1});
   }
-  return enqueue_eval(q, c->eb, c->stream, e0, e1);
+  return enqueue_eval(q, c->eb, st, e0, e1);
 }
 
-int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme) {
+int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme, hipStream_t st) {
   StageArgs a;
   a.nt = c->nt;
   a.nv = (int)c->nv;
@@ -1144,21 +1276,42 @@ int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme) {
   a.wy = c->wy;
   a.wz = c->wz;
   a.wzeta = c->wzeta;
-  hipLaunchKernelGGL(k_stage, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, c->stream, a);
+  hipLaunchKernelGGL(k_stage, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, st, a);
   LPM_HIP(hipGetLastError());
   return LPM_OK;
 }
 
-// One advance_timestep (src/LpmBVERK4_Impl.hpp:42-130) queued on the stream.
-int ctx_enqueue_step(lpm_bve_ctx* c, double dt, double omega, int scheme, int prec) {
+// Before stepping: absolute vorticity for the Lagrangian scheme and a valid
+// entry velocity for (prec, form), queued on `st`.
+int ctx_prepare_step(lpm_bve_ctx* c, double omega, int scheme, int prec, hipStream_t st) {
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  if (scheme == LPM_SCHEME_LAGR && !(c->absv_valid && c->absv_omega == omega)) {
+    hipLaunchKernelGGL(k_absv, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, st, c->nt, omega, c->zeta, c->z,
+                       c->absv);
+    LPM_HIP(hipGetLastError());
+    c->absv_valid = true;
+    c->absv_omega = omega;
+  }
+  if (!c->vel_valid || (c->vel_prec >= 0 && (c->vel_prec != prec || c->vel_form != form))) {
+    int rc = ctx_eval(c, c->x, c->zeta, prec, form, false, st);
+    if (rc) return rc;
+    c->vel_valid = true;
+    c->vel_prec = prec;
+    c->vel_form = form;
+  }
+  return ensure_eval_buffers(c, ctx_shape(c));
+}
+
+// One advance_timestep (src/LpmBVERK4_Impl.hpp:42-130) queued on `st`.
+int ctx_enqueue_step(lpm_bve_ctx* c, double dt, double omega, int scheme, int prec, hipStream_t st) {
   const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
   int rc;
   for (int stage = 1; stage <= 3; ++stage) {
-    if ((rc = ctx_stage(c, stage, dt, omega, scheme))) return rc;
-    if ((rc = ctx_eval(c, c->wx, c->wzeta, prec, form, false))) return rc;
+    if ((rc = ctx_stage(c, stage, dt, omega, scheme, st))) return rc;
+    if ((rc = ctx_eval(c, c->wx, c->wzeta, prec, form, false, st))) return rc;
   }
-  if ((rc = ctx_stage(c, 4, dt, omega, scheme))) return rc;
-  return ctx_eval(c, c->x, c->zeta, prec, form, false);
+  if ((rc = ctx_stage(c, 4, dt, omega, scheme, st))) return rc;
+  return ctx_eval(c, c->x, c->zeta, prec, form, false, st);
 }
 
 }  // namespace
@@ -1325,7 +1478,7 @@ int lpm_bve_solve(lpm_bve_ctx* c, int prec, int scheme, int want_psi) {
   if (scheme != LPM_SCHEME_REF && scheme != LPM_SCHEME_LAGR) return fail(LPM_ERR_ARG, "lpm_bve_solve: bad scheme");
   LPM_HIP(hipSetDevice(c->device));
   const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
-  int rc = ctx_eval(c, c->x, c->zeta, prec, form, want_psi != 0);
+  int rc = ctx_eval(c, c->x, c->zeta, prec, form, want_psi != 0, c->stream);
   if (rc) return rc;
   c->vel_valid = true;
   c->vel_prec = prec;
@@ -1345,28 +1498,15 @@ int lpm_bve_advance(lpm_bve_ctx* c, double dt, double omega, int scheme, int pre
   LPM_HIP(hipSetDevice(c->device));
   const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
   int rc;
-  if (scheme == LPM_SCHEME_LAGR && !(c->absv_valid && c->absv_omega == omega)) {
-    hipLaunchKernelGGL(k_absv, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, c->stream, c->nt, omega, c->zeta, c->z,
-                       c->absv);
-    LPM_HIP(hipGetLastError());
-    c->absv_valid = true;
-    c->absv_omega = omega;
-  }
-  if (!c->vel_valid || (c->vel_prec >= 0 && (c->vel_prec != prec || c->vel_form != form))) {
-    if ((rc = ctx_eval(c, c->x, c->zeta, prec, form, false))) return rc;
-    c->vel_valid = true;
-    c->vel_prec = prec;
-    c->vel_form = form;
-  }
+  if ((rc = ctx_prepare_step(c, omega, scheme, prec, c->stream))) return rc;
   const Shape sh = ctx_shape(c);
-  if ((rc = ensure_eval_buffers(c, sh))) return rc;
   if (use_graph && !c->timing && nsteps > 0) {
     lpm_bve_ctx::GraphKey key{dt, omega, scheme, prec, sh.R, sh.splits, sh.near_factor, sh.lds};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       hipGraph_t g;
       LPM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      rc = ctx_enqueue_step(c, dt, omega, scheme, prec);
+      rc = ctx_enqueue_step(c, dt, omega, scheme, prec, c->stream);
       hipError_t ce = hipStreamEndCapture(c->stream, &g);
       if (rc) return rc;
       if (ce != hipSuccess) return fail(LPM_ERR_HIP, std::string("step capture failed: ") + hipGetErrorString(ce));
@@ -1379,7 +1519,7 @@ int lpm_bve_advance(lpm_bve_ctx* c, double dt, double omega, int scheme, int pre
     for (int s = 0; s < nsteps; ++s) LPM_HIP(hipGraphLaunch(it->second, c->stream));
   } else {
     for (int s = 0; s < nsteps; ++s)
-      if ((rc = ctx_enqueue_step(c, dt, omega, scheme, prec))) return rc;
+      if ((rc = ctx_enqueue_step(c, dt, omega, scheme, prec, c->stream))) return rc;
   }
   c->vel_prec = prec;
   c->vel_form = form;
@@ -1452,8 +1592,8 @@ int lpm_bve_sizes(lpm_bve_ctx* c, int64_t* s) {
 
 int lpm_bve_set_launch(lpm_bve_ctx* c, int targets_per_lane, int source_splits) {
   if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_launch: null context");
-  if (targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4)
-    return fail(LPM_ERR_ARG, "lpm_bve_set_launch: targets_per_lane must be 1, 2 or 4");
+  if (targets_per_lane != 0 && targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4)
+    return fail(LPM_ERR_ARG, "lpm_bve_set_launch: targets_per_lane must be 0 (automatic), 1, 2 or 4");
   if (source_splits < 0) return fail(LPM_ERR_ARG, "lpm_bve_set_launch: source_splits < 0");
   c->R_req = targets_per_lane;
   c->splits_req = source_splits;
@@ -1506,6 +1646,441 @@ int lpm_bve_last_pair_timing(lpm_bve_ctx* c, int* launches, double* mean_ms) {
 int lpm_bve_enable_timing(lpm_bve_ctx* c, int on) {
   if (!c) return fail(LPM_ERR_ARG, "lpm_bve_enable_timing: null context");
   c->timing = on != 0;
+  return ok();
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// ensemble: independent problems advanced together, one persistent pair
+// kernel per evaluation over all members' (target block, source split) items
+// ===========================================================================
+struct lpm_bve_ensemble {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<lpm_bve_ctx*> members;
+  int blocks_per_cu = 0;  // persistent grid: 0 = occupancy limit
+  // plan (rebuilt when membership, generations or the step kind change)
+  bool plan_valid = false;
+  std::vector<uint64_t> plan_generations;
+  int plan_prec = -1, plan_form = -1;
+  std::vector<int> plan_R;                 // distinct R values, one launch each
+  std::vector<int> plan_nitems;            // items per R launch
+  std::vector<int> plan_grid;              // persistent workgroups per R launch
+  std::vector<lpm::dev::EnsItem*> d_items;  // per R launch
+  lpm::dev::EnsMember* d_mem_w = nullptr;  // descriptors: stage arguments as targets
+  lpm::dev::EnsMember* d_mem_x = nullptr;  // descriptors: state as targets
+  int* d_counters = nullptr;               // one per persistent launch in a step
+  std::vector<void*> plan_allocs;
+  std::map<std::tuple<double, double, int, int>, hipGraphExec_t> graphs;
+  hipEvent_t ev = nullptr;
+};
+
+namespace lpm {
+namespace {
+
+void ens_drop_plan(lpm_bve_ensemble* e) {
+  for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+  e->graphs.clear();
+  for (void* p : e->plan_allocs) (void)hipFree(p);
+  e->plan_allocs.clear();
+  e->d_items.clear();
+  e->plan_R.clear();
+  e->plan_nitems.clear();
+  e->plan_grid.clear();
+  e->d_mem_w = e->d_mem_x = nullptr;
+  e->d_counters = nullptr;
+  e->plan_valid = false;
+}
+
+bool ens_plan_current(const lpm_bve_ensemble* e, int prec, int form) {
+  if (!e->plan_valid || e->plan_prec != prec || e->plan_form != form) return false;
+  if (e->plan_generations.size() != e->members.size()) return false;
+  for (size_t m = 0; m < e->members.size(); ++m)
+    if (e->plan_generations[m] != e->members[m]->generation) return false;
+  return true;
+}
+
+int ens_grid(lpm_bve_ensemble* e, int R, int prec, int form, int nitems);
+
+// Build the flattened work lists and member descriptors for one kind of
+// evaluation (precision and kernel form).  Items are ordered by decreasing
+// cost so that the persistent workgroups take the long items first.
+int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
+  ens_drop_plan(e);
+  const size_t M = e->members.size();
+  std::vector<EnsMember> hw(M), hx(M);
+  std::map<int, std::vector<std::pair<long, EnsItem>>> byR;
+  for (size_t m = 0; m < M; ++m) {
+    lpm_bve_ctx* c = e->members[m];
+    for (int which = 0; which < 2; ++which) {
+      EvalRequest q;
+      int rc = ctx_request(c, which ? c->x : c->wx, which ? c->zeta : c->wzeta, prec, form, false, q);
+      if (rc) return rc;
+      EnsMember& d = which ? hx[m] : hw[m];
+      d.a = make_pair_args(q, c->eb);
+      d.src32 = c->eb.src32;
+      d.ctr = c->eb.ctr;
+      d.trad = c->eb.trad;
+      d.src64 = c->eb.src64;
+      if (which) {
+        const Shape& sh = q.shape;
+        const int nbx = cdiv(c->nt, (long)kBlock * sh.R);
+        for (int by = 0; by < sh.splits; ++by) {
+          const int tiles = std::min(sh.tiles_per_split, sh.ntiles - by * sh.tiles_per_split);
+          for (int bx = 0; bx < nbx; ++bx) {
+            const int tgts = std::min(kBlock * sh.R, c->nt - bx * kBlock * sh.R);
+            byR[sh.R].push_back({(long)tiles * tgts, EnsItem{(int)m, bx, by}});
+          }
+        }
+      }
+    }
+  }
+  auto alloc = [&](void** p, size_t n) -> int {
+    if (hipMalloc(p, std::max<size_t>(n, 256)) != hipSuccess) return fail(LPM_ERR_NOMEM, "ensemble plan allocation");
+    e->plan_allocs.push_back(*p);
+    return LPM_OK;
+  };
+  int rc;
+  if ((rc = alloc((void**)&e->d_mem_w, sizeof(EnsMember) * M))) return rc;
+  if ((rc = alloc((void**)&e->d_mem_x, sizeof(EnsMember) * M))) return rc;
+  LPM_HIP(hipMemcpy(e->d_mem_w, hw.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice));
+  LPM_HIP(hipMemcpy(e->d_mem_x, hx.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice));
+  for (auto& kv : byR) {
+    auto& v = kv.second;
+    std::stable_sort(v.begin(), v.end(), [](const std::pair<long, EnsItem>& a, const std::pair<long, EnsItem>& b) {
+      return a.first > b.first;
+    });
+    std::vector<EnsItem> items;
+    items.reserve(v.size());
+    for (auto& pr : v) items.push_back(pr.second);
+    EnsItem* d = nullptr;
+    if ((rc = alloc((void**)&d, sizeof(EnsItem) * items.size()))) return rc;
+    LPM_HIP(hipMemcpy(d, items.data(), sizeof(EnsItem) * items.size(), hipMemcpyHostToDevice));
+    e->plan_R.push_back(kv.first);
+    e->plan_nitems.push_back((int)items.size());
+    e->plan_grid.push_back(ens_grid(e, kv.first, prec, form, (int)items.size()));
+    e->d_items.push_back(d);
+  }
+  if ((rc = alloc((void**)&e->d_counters, sizeof(int) * 4 * std::max<size_t>(1, e->plan_R.size())))) return rc;
+  e->plan_generations.clear();
+  for (auto* c : e->members) e->plan_generations.push_back(c->generation);
+  e->plan_prec = prec;
+  e->plan_form = form;
+  e->plan_valid = true;
+  return LPM_OK;
+}
+
+template <int R>
+int ens_launch_R(lpm_bve_ensemble* e, int k, const EnsMember* mem, int* counter, int prec, int form, int grid) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, e->stream, counter);
+  if (prec == LPM_PREC_FP32C)
+    hipLaunchKernelGGL((k_pair_f32c_ens<R>), dim3(grid), dim3(kBlock), 0, e->stream, mem, e->d_items[k],
+                       e->plan_nitems[k], counter);
+  else if (form == FORM_DOT)
+    hipLaunchKernelGGL((k_pair_f64_ens<R, FORM_DOT>), dim3(grid), dim3(kBlock), 0, e->stream, mem, e->d_items[k],
+                       e->plan_nitems[k], counter);
+  else
+    hipLaunchKernelGGL((k_pair_f64_ens<R, FORM_CHORD>), dim3(grid), dim3(kBlock), 0, e->stream, mem,
+                       e->d_items[k], e->plan_nitems[k], counter);
+  LPM_HIP(hipGetLastError());
+  return LPM_OK;
+}
+
+template <int R>
+const void* ens_kernel(int prec, int form) {
+  if (prec == LPM_PREC_FP32C) return (const void*)&k_pair_f32c_ens<R>;
+  if (form == FORM_DOT) return (const void*)&k_pair_f64_ens<R, FORM_DOT>;
+  return (const void*)&k_pair_f64_ens<R, FORM_CHORD>;
+}
+
+int ens_grid(lpm_bve_ensemble* e, int R, int prec, int form, int nitems) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return 256;
+  int per_cu = e->blocks_per_cu;
+  if (per_cu <= 0) {
+    const void* k = R == 1 ? ens_kernel<1>(prec, form) : R == 4 ? ens_kernel<4>(prec, form) : ens_kernel<2>(prec, form);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+  }
+  return std::max(1, std::min(nitems, per_cu * prop.multiProcessorCount));
+}
+
+// One evaluation of every member: prep per member, one persistent pair
+// launch per distinct R, finalize per member.  which: 0 = stage arguments,
+// 1 = state.
+int ens_eval(lpm_bve_ensemble* e, int which, int prec, int form, int pass) {
+  int rc;
+  std::vector<EvalRequest> qs(e->members.size());
+  for (size_t m = 0; m < e->members.size(); ++m) {
+    lpm_bve_ctx* c = e->members[m];
+    if ((rc = ctx_request(c, which ? c->x : c->wx, which ? c->zeta : c->wzeta, prec, form, false, qs[m]))) return rc;
+    if ((rc = enqueue_prep(qs[m], c->eb, e->stream))) return rc;
+  }
+  const EnsMember* mem = which ? e->d_mem_x : e->d_mem_w;
+  for (size_t k = 0; k < e->plan_R.size(); ++k) {
+    int* counter = e->d_counters + 4 * k + pass;
+    const int R = e->plan_R[k];
+    const int grid = e->plan_grid[k];
+    if (R == 1) rc = ens_launch_R<1>(e, (int)k, mem, counter, prec, form, grid);
+    else if (R == 4) rc = ens_launch_R<4>(e, (int)k, mem, counter, prec, form, grid);
+    else rc = ens_launch_R<2>(e, (int)k, mem, counter, prec, form, grid);
+    if (rc) return rc;
+  }
+  for (size_t m = 0; m < e->members.size(); ++m)
+    if ((rc = enqueue_finalize(qs[m], e->members[m]->eb, e->stream))) return rc;
+  return LPM_OK;
+}
+
+int ens_enqueue_step(lpm_bve_ensemble* e, double dt, double omega, int scheme, int prec) {
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  int rc;
+  for (int stage = 1; stage <= 4; ++stage) {
+    for (auto* c : e->members)
+      if ((rc = ctx_stage(c, stage, dt, omega, scheme, e->stream))) return rc;
+    if ((rc = ens_eval(e, stage == 4 ? 1 : 0, prec, form, stage - 1))) return rc;
+  }
+  return LPM_OK;
+}
+
+}  // namespace
+}  // namespace lpm
+
+extern "C" {
+
+int lpm_ens_create(int device, lpm_bve_ensemble** out) {
+  if (!out) return fail(LPM_ERR_ARG, "lpm_ens_create: null output");
+  int n = 0;
+  LPM_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(LPM_ERR_ARG, "lpm_ens_create: no such device");
+  LPM_HIP(hipSetDevice(device));
+  auto* e = new lpm_bve_ensemble();
+  e->device = device;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess) {
+    delete e;
+    return fail(LPM_ERR_HIP, "lpm_ens_create: stream/event creation failed");
+  }
+  *out = e;
+  return ok();
+}
+
+int lpm_ens_destroy(lpm_bve_ensemble* e) {
+  if (!e) return ok();
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->stream);
+  ens_drop_plan(e);
+  (void)hipEventDestroy(e->ev);
+  (void)hipStreamDestroy(e->stream);
+  delete e;
+  return ok();
+}
+
+int lpm_ens_add(lpm_bve_ensemble* e, lpm_bve_ctx* c) {
+  if (!e || !c) return fail(LPM_ERR_ARG, "lpm_ens_add: null argument");
+  if (!c->x) return fail(LPM_ERR_STATE, "lpm_ens_add: member has no state");
+  if (c->device != e->device) return fail(LPM_ERR_ARG, "lpm_ens_add: member is on another device");
+  if (std::find(e->members.begin(), e->members.end(), c) != e->members.end())
+    return fail(LPM_ERR_ARG, "lpm_ens_add: already a member");
+  e->members.push_back(c);
+  e->plan_valid = false;
+  return ok();
+}
+
+int lpm_ens_remove(lpm_bve_ensemble* e, lpm_bve_ctx* c) {
+  if (!e || !c) return fail(LPM_ERR_ARG, "lpm_ens_remove: null argument");
+  auto it = std::find(e->members.begin(), e->members.end(), c);
+  if (it == e->members.end()) return fail(LPM_ERR_ARG, "lpm_ens_remove: not a member");
+  LPM_HIP(hipSetDevice(e->device));
+  LPM_HIP(hipStreamSynchronize(e->stream));
+  e->members.erase(it);
+  e->plan_valid = false;
+  return ok();
+}
+
+int lpm_ens_size(lpm_bve_ensemble* e, int* n) {
+  if (!e || !n) return fail(LPM_ERR_ARG, "lpm_ens_size: null argument");
+  *n = (int)e->members.size();
+  return ok();
+}
+
+int lpm_ens_set_blocks_per_cu(lpm_bve_ensemble* e, int n) {
+  if (!e || n < 0) return fail(LPM_ERR_ARG, "lpm_ens_set_blocks_per_cu: bad argument");
+  LPM_HIP(hipSetDevice(e->device));
+  LPM_HIP(hipStreamSynchronize(e->stream));
+  e->blocks_per_cu = n;
+  ens_drop_plan(e);
+  return ok();
+}
+
+int lpm_ens_advance(lpm_bve_ensemble* e, double dt, double omega, int scheme, int prec, int nsteps, int use_graph) {
+  if (!e) return fail(LPM_ERR_ARG, "lpm_ens_advance: null ensemble");
+  if (scheme != LPM_SCHEME_REF && scheme != LPM_SCHEME_LAGR) return fail(LPM_ERR_ARG, "lpm_ens_advance: bad scheme");
+  if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32C) return fail(LPM_ERR_ARG, "lpm_ens_advance: bad precision");
+  if (prec == LPM_PREC_FP32C && scheme != LPM_SCHEME_LAGR)
+    return fail(LPM_ERR_ARG, "lpm_ens_advance: FP32C pair sums require the renormalised (LAGR) scheme");
+  if (nsteps < 0) return fail(LPM_ERR_ARG, "lpm_ens_advance: nsteps < 0");
+  if (e->members.empty() || nsteps == 0) return ok();
+  LPM_HIP(hipSetDevice(e->device));
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  // order after each member's earlier work, then prepare every member
+  for (auto* c : e->members) {
+    if (c->timing) return fail(LPM_ERR_STATE, "lpm_ens_advance: member has pair timing enabled");
+    LPM_HIP(hipEventRecord(e->ev, c->stream));
+    LPM_HIP(hipStreamWaitEvent(e->stream, e->ev, 0));
+  }
+  int rc;
+  for (auto* c : e->members)
+    if ((rc = ctx_prepare_step(c, omega, scheme, prec, e->stream))) return rc;
+  if (!ens_plan_current(e, prec, form))
+    if ((rc = ens_build_plan(e, prec, form))) return rc;
+  if (use_graph) {
+    const auto key = std::make_tuple(dt, omega, scheme, prec);
+    auto it = e->graphs.find(key);
+    if (it == e->graphs.end()) {
+      hipGraph_t g;
+      LPM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      rc = ens_enqueue_step(e, dt, omega, scheme, prec);
+      hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if (rc) return rc;
+      if (ce != hipSuccess) return fail(LPM_ERR_HIP, std::string("ensemble capture failed: ") + hipGetErrorString(ce));
+      hipGraphExec_t ge;
+      hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(LPM_ERR_HIP, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+      it = e->graphs.emplace(key, ge).first;
+    }
+    for (int s = 0; s < nsteps; ++s) LPM_HIP(hipGraphLaunch(it->second, e->stream));
+  } else {
+    for (int s = 0; s < nsteps; ++s)
+      if ((rc = ens_enqueue_step(e, dt, omega, scheme, prec))) return rc;
+  }
+  // later work on each member's own stream waits for the ensemble's
+  LPM_HIP(hipEventRecord(e->ev, e->stream));
+  for (auto* c : e->members) {
+    LPM_HIP(hipStreamWaitEvent(c->stream, e->ev, 0));
+    c->vel_valid = true;
+    c->vel_prec = prec;
+    c->vel_form = form;
+    c->psi_valid = false;
+  }
+  return ok();
+}
+
+int lpm_ens_synchronize(lpm_bve_ensemble* e) {
+  if (!e) return fail(LPM_ERR_ARG, "lpm_ens_synchronize: null ensemble");
+  LPM_HIP(hipSetDevice(e->device));
+  LPM_HIP(hipStreamSynchronize(e->stream));
+  return ok();
+}
+
+// ---------------------------------------------------------------------------
+// launch-shape autotuning, keyed on the number of targets N
+// ---------------------------------------------------------------------------
+int lpm_shape_rule_set(int64_t ntargets, int targets_per_lane, int source_splits) {
+  if (ntargets < 1 || (targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4) ||
+      source_splits < 0)
+    return fail(LPM_ERR_ARG, "lpm_shape_rule_set: bad arguments");
+  std::lock_guard<std::mutex> g(g_rule_mu);
+  g_rules[(int)ntargets] = ShapeRule{targets_per_lane, source_splits};
+  return ok();
+}
+
+int lpm_shape_rule_get(int64_t ntargets, int* targets_per_lane, int* source_splits) {
+  std::lock_guard<std::mutex> g(g_rule_mu);
+  auto it = g_rules.find((int)ntargets);
+  if (it == g_rules.end()) return fail(LPM_ERR_ARG, "lpm_shape_rule_get: no rule for this N");
+  if (targets_per_lane) *targets_per_lane = it->second.R;
+  if (source_splits) *source_splits = it->second.splits;
+  return ok();
+}
+
+int lpm_shape_rules_clear(void) {
+  std::lock_guard<std::mutex> g(g_rule_mu);
+  g_rules.clear();
+  return ok();
+}
+
+// Time one evaluation per candidate (targets per lane x source splits) on
+// this context's problem and record the fastest as the rule for its N.
+int lpm_bve_autotune(lpm_bve_ctx* c, int prec, int scheme, int reps, int* best_R, int* best_splits, double* best_ms) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_autotune: no state");
+  if (reps < 1) reps = 1;
+  LPM_HIP(hipSetDevice(c->device));
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  const int saved_R = c->R_req, saved_splits = c->splits_req;
+  const bool saved_timing = c->timing;
+  const int ntiles = std::max(1, cdiv(c->ns, kTile));
+  std::vector<std::pair<int, int>> cands;
+  for (int R : {1, 2, 4}) {
+    const Shape base = choose_shape(c->nt, std::max(c->ns, 1), R, 0);
+    std::vector<int> sp = {base.splits, 1, 2, 4, 8, 16};
+    std::sort(sp.begin(), sp.end());
+    sp.erase(std::unique(sp.begin(), sp.end()), sp.end());
+    for (int s : sp)
+      if (s <= ntiles) cands.push_back({R, s});
+  }
+  double best = 1e300;
+  int bR = 2, bS = 0;
+  c->timing = true;
+  for (auto& cd : cands) {
+    c->R_req = cd.first;
+    c->splits_req = cd.second;
+    for (auto& ev : c->timing_events) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    c->timing_events.clear();
+    int rc = ctx_eval(c, c->x, c->zeta, prec, form, false, c->stream);  // warm
+    for (int r = 0; r < reps && !rc; ++r) rc = ctx_eval(c, c->x, c->zeta, prec, form, false, c->stream);
+    if (rc) {
+      c->R_req = saved_R;
+      c->splits_req = saved_splits;
+      c->timing = saved_timing;
+      return rc;
+    }
+    LPM_HIP(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (size_t k = 1; k < c->timing_events.size(); ++k) {
+      float ms = 0.0f;
+      LPM_HIP(hipEventElapsedTime(&ms, c->timing_events[k].first, c->timing_events[k].second));
+      tot += ms;
+    }
+    const double mean = tot / reps;
+    if (mean < best) {
+      best = mean;
+      bR = cd.first;
+      bS = cd.second;
+    }
+  }
+  for (auto& ev : c->timing_events) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
+  c->timing_events.clear();
+  c->timing = saved_timing;
+  c->R_req = saved_R;
+  c->splits_req = saved_splits;
+  {
+    std::lock_guard<std::mutex> g(g_rule_mu);
+    g_rules[c->nt] = ShapeRule{bR, bS};
+  }
+  // the trial evaluations overwrote the velocity with the last candidate's
+  // rounding; the next step recomputes it with the chosen shape
+  c->vel_valid = false;
+  if (best_R) *best_R = bR;
+  if (best_splits) *best_splits = bS;
+  if (best_ms) *best_ms = best;
+  return ok();
+}
+
+// Device memory one context needs for a problem of this size (bytes).
+int lpm_bve_state_bytes(int64_t nv, int64_t nactive, int64_t* bytes) {
+  if (nv < 0 || nactive < 1 || !bytes) return fail(LPM_ERR_ARG, "lpm_bve_state_bytes: bad arguments");
+  const int64_t nt = nv + nactive;
+  const Shape sh = choose_shape((int)nt, (int)nactive, 0, 0);
+  const BufferPlan bp = plan_buffers((int)nt, sh);
+  const int64_t state = 8 * nt * (3 + 3 + 3 + 5 + 16) + 4 * (nt + nactive);
+  *bytes = state + (int64_t)(bp.src64 + bp.src32 + bp.ctr + bp.trad + bp.srcq + bp.part) + 8 * 256;
   return ok();
 }
 

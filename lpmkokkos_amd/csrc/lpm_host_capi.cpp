@@ -86,6 +86,15 @@ int lpm_ic_gauss_params(uint64_t seed, int nvort, double amp_scale, double* cent
   return ok();
 }
 
+int lpm_rng_draws(uint64_t seed, int64_t nuniform, double* uniform, int64_t nnormal, double* normal) {
+  if (nuniform < 0 || nnormal < 0 || (nuniform && !uniform) || (nnormal && !normal))
+    return fail(LPM_ERR_ARG, "lpm_rng_draws: bad arguments");
+  Xoshiro256ss rng(seed);
+  for (int64_t i = 0; i < nuniform; ++i) uniform[i] = rng.uniform();
+  for (int64_t i = 0; i < nnormal; ++i) normal[i] = rng.normal();
+  return ok();
+}
+
 int lpm_ic_eval(const double* xyz, int64_t n, double* zeta, double sbr, int rh_n, double rh_w, double rh_K,
                 int nvort, const double* centers, const double* amps, double sigma) {
   if (n < 0 || nvort < 0 || (nvort > 0 && (!centers || !amps)))

@@ -1,0 +1,229 @@
+"""Serving form: a continuous-batching request loop for BVE forecasts.
+
+Requests (mesh level, initial field, number of RK4 steps) arrive by a seeded
+Poisson process.  Between steps the loop admits every request that has
+arrived while a slot and the device-memory budget allow, keeps each admitted
+request's particle state resident in HBM (one context per request), advances
+all of them together -- one ensemble step is four persistent pair kernels over
+the flattened work of every active request -- and retires requests that have
+completed their steps, freeing their memory for the next arrivals.  Host
+preparation of new arrivals (mesh from the per-level cache, initial field,
+upload) overlaps the device step in flight.
+
+Each request's result is bit-identical to running it alone: a member keeps
+the decomposition it has alone and writes only its own partial slots.
+
+Reported: completion latency (finish - arrival) p50 / p99 and aggregate
+pair interactions per second over the run.
+
+    python -m lpmkokkos_amd.serve --requests 2000 --budget-gb 200
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import PREC_FP32C, SCHEME_LAGR
+from .bve import Context
+from .ensemble import Ensemble, MemberSpec, leaf_mesh, make_member, rng_draws, state_bytes
+
+LEVEL_PROBS = {4: 0.30, 5: 0.30, 6: 0.20, 7: 0.15, 8: 0.05}
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds after the start of the run
+    spec: MemberSpec
+    steps: int = 20
+    # filled by the server
+    ctx: Context | None = None
+    bytes: int = 0
+    admitted: float = math.nan
+    finished: float = math.nan
+    steps_done: int = 0
+    pairs: float = 0.0
+    result: dict | None = field(default=None, repr=False)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def pairs_per_step(level: int) -> float:
+    nl = 20 * 4 ** level
+    nv = 2 + 10 * 4 ** level
+    return 4.0 * ((nv + nl) * nl - nl)
+
+
+def make_requests(n: int, rate: float, seed: int = 4, level_probs=None, steps: int = 20) -> list[Request]:
+    """n requests with exponential inter-arrival times of mean 1/rate; level
+    drawn from level_probs; RH wavenumber uniform on 2..6 and 4..16 Gaussian
+    vortices (amplitude N(0, 0.05), sigma 0.15), all from `seed`."""
+    probs = level_probs or LEVEL_PROBS
+    levels = sorted(probs)
+    cum = np.cumsum([probs[l] for l in levels])
+    cum /= cum[-1]
+    u, _ = rng_draws(seed, 5 * n)
+    t = 0.0
+    reqs = []
+    for i in range(n):
+        t += -math.log(1.0 - u[5 * i]) / rate
+        lev = levels[int(np.searchsorted(cum, u[5 * i + 1], side="right"))] if u[5 * i + 1] < cum[-1] else levels[-1]
+        spec = MemberSpec(level=lev, k_factor=0.95 + 0.1 * u[5 * i + 2], rh_n=2 + int(5 * u[5 * i + 3]),
+                          nvort=4 + int(13 * u[5 * i + 4]), amp=0.05, sigma=0.15, vortex_seed=100000 * seed + i)
+        reqs.append(Request(rid=i, arrival=t, spec=spec, steps=steps))
+    return reqs
+
+
+def request_bytes(spec: MemberSpec) -> int:
+    nv = 2 + 10 * 4 ** spec.level
+    return state_bytes(nv, 20 * 4 ** spec.level)
+
+
+class Server:
+    def __init__(self, device: int = 0, budget_bytes: float = 200e9, max_active: int = 256, dt: float = 0.005,
+                 omega: float = 2 * math.pi, prec: int = PREC_FP32C, keep_results: bool = False,
+                 log=None):
+        self.device = device
+        self.budget = budget_bytes
+        self.max_active = max_active
+        self.dt, self.omega, self.prec = dt, omega, prec
+        self.keep_results = keep_results
+        self.ens = Ensemble(device)
+        self.active: list[Request] = []
+        self.used = 0
+        self.log = log or (lambda s: None)
+
+    def _admit(self, r: Request, now: float):
+        r.ctx = make_member(r.spec, self.device)
+        r.bytes = request_bytes(r.spec)
+        self.used += r.bytes
+        r.admitted = now
+        self.ens.add(r.ctx)
+        self.active.append(r)
+
+    def _retire(self, r: Request, now: float):
+        if self.keep_results:
+            r.result = r.ctx.get_state()
+        else:
+            r.ctx.synchronize()
+        r.finished = now
+        self.ens.remove(r.ctx)
+        r.ctx.close()
+        r.ctx = None
+        self.used -= r.bytes
+        self.active.remove(r)
+
+    def run(self, requests: list[Request]) -> dict:
+        for r in requests:
+            if request_bytes(r.spec) > self.budget:
+                raise ValueError(f"request {r.rid} (level {r.spec.level}) exceeds the memory budget")
+        pending = sorted(requests, key=lambda r: r.arrival)
+        done: list[Request] = []
+        t0 = time.perf_counter()
+        busy = 0.0
+        pairs = 0.0
+        steps = 0
+        while pending or self.active:
+            now = time.perf_counter() - t0
+            # admit arrived requests in arrival order while a slot and memory allow
+            while (pending and pending[0].arrival <= now and len(self.active) < self.max_active
+                   and self.used + request_bytes(pending[0].spec) <= self.budget):
+                self._admit(pending.pop(0), now)
+            if not self.active:
+                wait = pending[0].arrival - now
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                continue
+            ts = time.perf_counter()
+            self.ens.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=1)
+            # prepare the next arrivals' meshes while the step runs
+            lookahead = time.perf_counter() - t0 + 0.5
+            for r in pending[:4]:
+                if r.arrival <= lookahead:
+                    leaf_mesh(r.spec.level)
+            self.ens.synchronize()
+            te = time.perf_counter()
+            busy += te - ts
+            steps += 1
+            now = te - t0
+            for r in list(self.active):
+                r.steps_done += 1
+                inc = pairs_per_step(r.spec.level)
+                r.pairs += inc
+                pairs += inc
+                if r.steps_done >= r.steps:
+                    self._retire(r, now)
+                    done.append(r)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in done])
+        return {
+            "requests": len(done),
+            "wall_s": wall,
+            "device_busy_s": busy,
+            "ensemble_steps": steps,
+            "pairs": pairs,
+            "pairs_per_s": pairs / wall,
+            "pairs_per_busy_s": pairs / busy if busy else None,
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_mean_s": float(lat.mean()),
+            "levels": {int(l): int(sum(1 for r in done if r.spec.level == l)) for l in sorted({r.spec.level for r in done})},
+        }
+
+    def close(self):
+        self.ens.close()
+
+
+def measure_service_times(levels, steps: int = 20, device: int = 0) -> dict:
+    """Device seconds for one request of each level, run alone (capacity
+    calibration)."""
+    out = {}
+    for lev in levels:
+        c = make_member(MemberSpec(level=lev, vortex_seed=7), device)
+        c.advance(0.005, 2 * math.pi, SCHEME_LAGR, PREC_FP32C, nsteps=1)
+        c.synchronize()
+        n = 2 if lev >= 8 else 4
+        t = time.perf_counter()
+        c.advance(0.005, 2 * math.pi, SCHEME_LAGR, PREC_FP32C, nsteps=n)
+        c.synchronize()
+        out[lev] = (time.perf_counter() - t) / n * steps
+        c.close()
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--load", type=float, default=0.8, help="arrival rate as a fraction of measured capacity")
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-level", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=4)
+    args = ap.parse_args(argv)
+    probs = {l: p for l, p in LEVEL_PROBS.items() if l <= args.max_level}
+    tot = sum(probs.values())
+    probs = {l: p / tot for l, p in probs.items()}
+    svc = measure_service_times(sorted(probs), steps=args.steps)
+    mean_service = sum(probs[l] * svc[l] for l in probs)
+    rate = args.load / mean_service
+    reqs = make_requests(args.requests, rate, seed=args.seed, level_probs=probs, steps=args.steps)
+    srv = Server(budget_bytes=args.budget_gb * 1e9, log=lambda s: print(s, file=sys.stderr, flush=True))
+    res = srv.run(reqs)
+    srv.close()
+    res.update({"arrival_rate_per_s": rate, "service_s_by_level": svc, "load": args.load,
+                "level_probs": probs, "budget_gb": args.budget_gb})
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

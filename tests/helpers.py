@@ -44,3 +44,53 @@ def normwise(a, b):
     a = np.asarray(a).reshape(len(b), -1)
     b = np.asarray(b).reshape(len(b), -1)
     return float(np.linalg.norm(a - b, axis=1).max() / np.linalg.norm(b, axis=1).max())
+
+
+class DeviceArray:
+    """Device buffer allocated through the HIP runtime liblpm_bve.so already
+    loaded (one HIP runtime per process), for the functor entry points."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes as C
+
+            import lpmkokkos_amd
+            lpmkokkos_amd.lib()  # loads libamdhip64 as the library's dependency
+            h = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host: np.ndarray):
+        import ctypes as C
+        self.shape, self.dtype = host.shape, host.dtype
+        self.nbytes = host.nbytes
+        p = C.c_void_p()
+        assert self.hip().hipMalloc(C.byref(p), max(self.nbytes, 8)) == 0
+        self.ptr = p.value
+        h = np.ascontiguousarray(host)
+        assert self.hip().hipMemcpy(self.ptr, h.ctypes.data, self.nbytes, 1) == 0  # host to device
+
+    @classmethod
+    def zeros(cls, shape, dtype=np.float64):
+        return cls(np.zeros(shape, dtype))
+
+    def data_ptr(self):
+        return self.ptr
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        assert self.hip().hipDeviceSynchronize() == 0
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None

@@ -86,3 +86,27 @@ def test_rh_phase_speed():
     ic = L.RossbyHaurwitzWave(4, 0.1, 0.1)
     # nu = (n(n+3) w - 2 Omega) / ((n+1)(n+2)) = (2.8 - 4 pi) / 30
     assert abs(ic.phase_speed(2 * np.pi) - (2.8 - 4 * np.pi) / 30) < 1e-15
+
+
+def test_config3_members_build():
+    from lpmkokkos_amd import ensemble as E
+    sp = E.ensemble_config(seed=3, n=64)
+    assert len(sp) == 64 and {s.level for s in sp} <= {3, 4, 5, 6, 7}
+    assert all(0.95 <= s.k_factor <= 1.05 for s in sp)
+    assert len({s.vortex_seed for s in sp}) == 64
+    u, n = E.rng_draws(9, 1000, 1000)
+    assert 0 <= u.min() and u.max() < 1 and abs(n.mean()) < 0.15
+    assert E.state_bytes(655362, 1310720) > 29 * 8 * (655362 + 1310720)
+
+
+def test_request_generator():
+    from lpmkokkos_amd import serve as S
+    reqs = S.make_requests(2000, rate=0.5, seed=4)
+    lv = np.array([r.spec.level for r in reqs])
+    frac = {l: float(np.mean(lv == l)) for l in (4, 5, 6, 7, 8)}
+    for l, p in S.LEVEL_PROBS.items():
+        assert abs(frac[l] - p) < 0.04, (l, frac[l])
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0) and abs(np.mean(np.diff(arr)) - 2.0) < 0.2
+    assert {r.spec.rh_n for r in reqs} == {2, 3, 4, 5, 6}
+    assert min(r.spec.nvort for r in reqs) == 4 and max(r.spec.nvort for r in reqs) == 16

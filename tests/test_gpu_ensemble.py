@@ -1,0 +1,97 @@
+"""Batching form: ensemble members give the same bits as the same problems
+advanced alone, across precisions, graph replay, membership changes and
+autotuned launch shapes."""
+import math
+
+import numpy as np
+import pytest
+
+import lpmkokkos_amd as L
+from lpmkokkos_amd import ensemble as E
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * math.pi
+KEYS = ("vx", "vzeta", "vu", "fx", "fzeta", "fu")
+
+
+def specs():
+    return [E.MemberSpec(level=lv, k_factor=kf, vortex_seed=50 + i)
+            for i, (lv, kf) in enumerate([(2, 1.0), (3, 0.97), (4, 1.03), (3, 1.01), (5, 0.99), (2, 1.02)])]
+
+
+def same(a, b):
+    sa, sb = a.get_state(), b.get_state()
+    return all(np.array_equal(sa[k], sb[k]) for k in KEYS)
+
+
+@pytest.mark.parametrize("prec,use_graph", [(L.PREC_FP32C, True), (L.PREC_FP64, False), (L.PREC_FP64, True)])
+def test_members_match_alone(prec, use_graph):
+    sp = specs()
+    alone = [E.make_member(s) for s in sp]
+    mixed = [E.make_member(s) for s in sp]
+    ens = E.Ensemble(0)
+    for c in mixed:
+        ens.add(c)
+    for c in alone:
+        c.advance(0.005, TWO_PI, L.SCHEME_LAGR, prec, nsteps=3, use_graph=use_graph)
+    ens.advance(0.005, TWO_PI, L.SCHEME_LAGR, prec, nsteps=3, use_graph=use_graph)
+    for a, b in zip(alone, mixed):
+        assert same(a, b)
+    # the ensemble result is a real step: positions moved and stayed on the sphere
+    s = mixed[2].get_state()
+    assert np.abs(np.linalg.norm(s["fx"], axis=1) - 1).max() < 1e-15
+    ens.close()
+    for c in alone + mixed:
+        c.close()
+
+
+def test_membership_changes():
+    """Remove and add members between steps (continuous batching)."""
+    sp = specs()
+    alone = [E.make_member(s) for s in sp]
+    mixed = [E.make_member(s) for s in sp]
+    ens = E.Ensemble(0)
+    for c in mixed[:4]:
+        ens.add(c)
+    ens.advance(0.005, TWO_PI, nsteps=2)
+    ens.remove(mixed[1])
+    ens.add(mixed[4])
+    ens.add(mixed[5])
+    ens.advance(0.005, TWO_PI, nsteps=1)
+    assert len(ens) == 5
+    steps = [3, 2, 3, 3, 1, 1]
+    for c, n in zip(alone, steps):
+        c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=n)
+    for a, b in zip(alone, mixed):
+        assert same(a, b)
+    ens.close()
+    for c in alone + mixed:
+        c.close()
+
+
+def test_autotuned_shape_is_shared():
+    """The autotuner's rule for N applies to every context of that N, alone
+    and in an ensemble."""
+    s = E.MemberSpec(level=4, vortex_seed=7)
+    a = E.make_member(s)
+    R, S, ms = E.autotune(a, reps=1)
+    # the tuned context itself steps like a fresh one
+    d = E.make_member(s)
+    a.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1)
+    d.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1)
+    assert same(a, d)
+    d.close()
+    nt = a.sizes()[3]
+    assert E.shape_rule(nt) == (R, S) and ms > 0
+    b = E.make_member(s)
+    c = E.make_member(s)
+    ens = E.Ensemble(0)
+    ens.add(c)
+    b.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=2)
+    ens.advance(0.005, TWO_PI, nsteps=2)
+    assert same(b, c)
+    L._lib.call("lpm_shape_rules_clear")
+    ens.close()
+    for x in (a, b, c):
+        x.close()

@@ -27,32 +27,30 @@ def make_ctx(lm, vz, fz):
 def test_functor_entry_points(omesh):
     """BVE{Vertex,Face}{Velocity,StreamFn} on caller-owned device arrays over all
     tree faces (masked loop), against the oracle at vertices and active faces."""
-    import torch
+    from helpers import DeviceArray
     m = omesh(4)
     fz = rh_field(m.fx)
-    dev = torch.device("cuda:0")
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    vx, fx, z, a, msk = t(m.vx), t(m.fx), t(fz), t(m.farea), t(m.fmask)
-    uv = torch.zeros_like(vx)
-    uf = torch.zeros_like(fx)
-    pv = torch.zeros(m.nv, dtype=torch.float64, device=dev)
-    pf = torch.zeros(m.nf, dtype=torch.float64, device=dev)
-    st = torch.cuda.current_stream().cuda_stream
+    vx, fx, z, a, msk = (DeviceArray(h) for h in (m.vx, m.fx, fz, m.farea, m.fmask))
+    uv = DeviceArray.zeros(m.vx.shape)
+    uf = DeviceArray.zeros(m.fx.shape)
+    pv = DeviceArray.zeros(m.nv)
+    pf = DeviceArray.zeros(m.nf)
     P = L._lib.ptr
-    L._lib.call("lpm_bve_vertex_velocity", P(uv), P(vx), m.nv, P(fx), P(z), P(a), P(msk), m.nf, st)
-    L._lib.call("lpm_bve_face_velocity", P(uf), P(fx), P(z), P(a), P(msk), m.nf, st)
-    L._lib.call("lpm_bve_vertex_streamfn", P(pv), P(vx), m.nv, P(fx), P(z), P(a), P(msk), m.nf, st)
-    L._lib.call("lpm_bve_face_streamfn", P(pf), P(fx), P(z), P(a), P(msk), m.nf, st)
-    torch.cuda.synchronize()
+    L._lib.call("lpm_bve_vertex_velocity", P(uv), P(vx), m.nv, P(fx), P(z), P(a), P(msk), m.nf, None)
+    L._lib.call("lpm_bve_face_velocity", P(uf), P(fx), P(z), P(a), P(msk), m.nf, None)
+    L._lib.call("lpm_bve_vertex_streamfn", P(pv), P(vx), m.nv, P(fx), P(z), P(a), P(msk), m.nf, None)
+    L._lib.call("lpm_bve_face_streamfn", P(pf), P(fx), P(z), P(a), P(msk), m.nf, None)
     lv = m.fmask == 0
     ov = O.velocity(m.vx, m.fx, fz, m.farea, m.fmask, False)
     of = O.velocity(m.fx, m.fx, fz, m.farea, m.fmask, True)
     opv = O.streamfn(m.vx, m.fx, fz, m.farea, m.fmask, False)
     opf = O.streamfn(m.fx, m.fx, fz, m.farea, m.fmask, True)
-    assert normwise(uv.cpu().numpy(), ov) < 1e-13
-    assert normwise(uf.cpu().numpy()[lv], of[lv]) < 1e-13
-    assert normwise(pv.cpu().numpy(), opv) < 1e-13
-    assert normwise(pf.cpu().numpy()[lv], opf[lv]) < 1e-13
+    assert normwise(uv.numpy(), ov) < 1e-13
+    assert normwise(uf.numpy()[lv], of[lv]) < 1e-13
+    assert normwise(pv.numpy(), opv) < 1e-13
+    assert normwise(pf.numpy()[lv], opf[lv]) < 1e-13
+    for d in (vx, fx, z, a, msk, uv, uf, pv, pf):
+        d.free()
 
 
 @pytest.mark.parametrize("level", [0, 2, 4])
