@@ -7,7 +7,9 @@
 // Solid-body rotation (default) has an exact solution: velocity Omega z x x
 // and rigid rotation of the Lagrangian coordinates; the driver prints the
 // area-weighted error norms (src/LpmErrorNorms.cpp:26-45) at t = 0 and at
-// tfinal and, unlike the reference, fails (exit 1) when they exceed bounds.
+// tfinal and, unlike the reference, fails (exit 1) when they exceed bounds
+// (loose for the reference step, which is not renormalised, tight for the
+// Lagrangian step).
 // The Rossby-Haurwitz case reports the measured vs analytic pattern speed.
 #include <chrono>
 #include <cmath>
@@ -178,7 +180,10 @@ int main(int argc, char* argv[]) {
       print_norms("velocity error at t=tfinal", nv);
       print_norms("position error at t=tfinal", np);
       std::printf("vorticity change at t=tfinal: linf = %g\n", vort);
-      ok = ok && nv.linf < 0.05 && np.linf < 0.01;
+      // the reference step drifts off the sphere (no renormalisation), so its
+      // rigid-rotation error grows with t; the Lagrangian step keeps it small
+      const bool lagr = scheme == LPM_SCHEME_LAGR;
+      ok = ok && nv.linf < (lagr ? 0.05 : 0.1) && np.linf < (lagr ? 0.01 : 0.2);
     } else {
       const Real phase1 = rh_phase(sphere.face_x, sphere.rel_vort_faces, sphere.face_mask, 4, 0.6);
       Real dphi = phase1 - phase0;
