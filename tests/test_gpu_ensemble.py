@@ -95,3 +95,22 @@ def test_autotuned_shape_is_shared():
     ens.close()
     for x in (a, b, c):
         x.close()
+
+
+def test_reference_scheme_members_match_alone():
+    """The reference step (face vorticity combination, no renormalisation)
+    inside an ensemble, FP64 with the literal 1 - x.y."""
+    sp = specs()[:4]
+    alone = [E.make_member(s) for s in sp]
+    mixed = [E.make_member(s) for s in sp]
+    ens = E.Ensemble(0)
+    for c in mixed:
+        ens.add(c)
+    for c in alone:
+        c.advance(0.005, TWO_PI, L.SCHEME_REF, L.PREC_FP64, nsteps=2)
+    ens.advance(0.005, TWO_PI, L.SCHEME_REF, L.PREC_FP64, nsteps=2)
+    for a, b in zip(alone, mixed):
+        assert same(a, b)
+    ens.close()
+    for c in alone + mixed:
+        c.close()

@@ -301,3 +301,66 @@ def test_fp64_staging_variants(omesh, lds):
     ps = O.streamfn_chord(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True)
     assert normwise(s["fpsi"], ps) < 1e-13
     c.close()
+
+
+@pytest.mark.parametrize("level", range(6))
+def test_gpu_reproduces_published_poisson_table(omesh, level):
+    """SpherePoisson's Y_5^4 problem (src/LpmSpherePoisson.hpp:428-475) through
+    the GPU stream-function solve: psi L-inf error at vertices and active faces
+    equals the reference's published table (scripts/poissonOutput.py:6-30) to
+    the printed six digits."""
+    from test_oracle_golden import POISSON_ICOS, sig6
+    m = omesh(level)
+    fv, _ = O.sphharm54(m.vx)
+    ff, _ = O.sphharm54(m.fx)
+    c = L.Context(0)
+    c.set_state(m.vx, fv, m.fx, ff, m.farea, m.fmask)
+    c.solve(L.PREC_FP64, L.SCHEME_REF, want_psi=True)
+    s = c.get_state()
+    lv = m.fmask == 0
+    ev = np.abs(s["vpsi"] - fv / 30).max()
+    ef = np.abs(s["fpsi"][lv] - ff[lv] / 30).max()
+    assert sig6(ev, POISSON_ICOS[level][1]) and sig6(ef, POISSON_ICOS[level][2]), (ev, ef)
+    c.close()
+
+
+def test_edge_cases(omesh):
+    """Faces-only problem (no vertices), a single active source, level 0."""
+    m = omesh(0)
+    fz = rh_field(m.fx)
+    c = L.Context(0)
+    c.set_state(np.zeros((0, 3)), np.zeros(0), m.fx, fz, m.farea, m.fmask)
+    c.solve(L.PREC_FP64, L.SCHEME_REF, want_psi=True)
+    s = c.get_state()
+    of = O.velocity(m.fx, m.fx, fz, m.farea, m.fmask, True)
+    assert normwise(s["fu"], of) < 1e-13
+    c.advance(0.01, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=2)
+    assert np.isfinite(c.get_state()["fx"]).all()
+    one = np.ones(m.nf, np.uint8)
+    one[5] = 0
+    c.set_state(m.vx, rh_field(m.vx), m.fx, fz, m.farea, one)
+    c.solve(L.PREC_FP32C, L.SCHEME_LAGR)
+    s = c.get_state()
+    assert np.all(s["fu"][5] == 0)  # the only source excludes itself
+    ov = O.velocity_chord(m.vx, m.fx, fz, m.farea, one, False)
+    assert normwise(s["vu"], ov) < 1e-6
+    c.close()
+
+
+def test_error_paths():
+    c = L.Context(0)
+    with pytest.raises(L.LpmError):
+        c.solve(L.PREC_FP64, L.SCHEME_REF)  # no state yet
+    m = O.Mesh(1)
+    with pytest.raises(L.LpmError):  # every face masked: no source
+        c.set_state(m.vx, np.zeros(m.nv), m.fx, np.zeros(m.nf), m.farea, np.ones(m.nf, np.uint8))
+    c.set_state(m.vx, np.zeros(m.nv), m.fx, np.zeros(m.nf), m.farea, m.fmask)
+    with pytest.raises(L.LpmError):  # FP32C needs the renormalised scheme
+        c.advance(0.01, TWO_PI, L.SCHEME_REF, L.PREC_FP32C)
+    with pytest.raises(L.LpmError):
+        c.set_launch(3, 0)
+    with pytest.raises(L.LpmError):
+        c.advance(float("nan"), TWO_PI)
+    c.close()
+    with pytest.raises(L.LpmError):
+        L.Context(99)
