@@ -26,13 +26,15 @@ constexpr double kLn2 = 0.69314718055994530941723212145817657;
 // scalar helpers
 // ---------------------------------------------------------------------------
 
-// Reciprocal: hardware estimate refined by two Newton steps (error ~1 ulp).
+// Reciprocal of a positive double in the FP32 normal range.  r = v_rcp_f32
+// estimate (relative error |e| < 2^-23, issued to the transcendental unit
+// beside the VALU); with e = 1 - a r (one FMA, nearly exact),
+// 1/a = r/(1 - e) = r (1 + e + e^2) + O(e^3 r), and e^3 < 2^-69, so two more
+// FMAs give the reciprocal to within the final rounding.
 __device__ __forceinline__ double rcp64(double a) {
-  double r = __builtin_amdgcn_rcp(a);
-  double e = fma(-a, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-a, r, 1.0);
-  return fma(r, e, r);
+  const double r = (double)__builtin_amdgcn_rcpf((float)a);
+  const double e = fma(-a, r, 1.0);
+  return fma(r, fma(e, e, e), r);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -44,24 +46,34 @@ __device__ __forceinline__ double wave_sum(double v) {
 // source preparation
 // ---------------------------------------------------------------------------
 
-// FP64 source records {x, y, z, q}.  Source j is target smap[j]; its area is
-// area[smap[j]].  FORM_DOT: q = (-zeta A)/(4 pi); FORM_CHORD: q = (-zeta A)/(2 pi).
-// Padding records have q = 0 at the origin (denominator 1 in both forms).
-template <int FORM>
-__global__ __launch_bounds__(kBlock) void k_prep_f64(double4* __restrict__ src, Pos3 pos,
-                                                      const double* __restrict__ zeta,
-                                                      const double* __restrict__ area,
-                                                      const int* __restrict__ smap, int ns, int ns_pad) {
+// FP64 source record (64 bytes).  Dot form (k_prep_f64_dot):
+// a = {y, q}, b = {q y, 0} with q = (-zeta A)/(4 pi).  Tile-local |y - x|^2/2
+// form (k_prep_f64_tiles): a = {d, |d|^2}, b = {q, q d} with d = y - c the
+// offset from the tile centre c and q = (-zeta A)/(2 pi).
+struct Src64 {
+  double4 a, b;
+};
+
+// Dot-form records.  Source j is target smap[j]; its area is area[smap[j]].
+// Padding records sit at the origin with q = 0 (denominator 1).
+__global__ __launch_bounds__(kBlock) void k_prep_f64_dot(Src64* __restrict__ src, Pos3 pos,
+                                                         const double* __restrict__ zeta,
+                                                         const double* __restrict__ area,
+                                                         const int* __restrict__ smap, int ns, int ns_pad) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= ns_pad) return;
+  Src64 r;
   if (j < ns) {
     const int t = smap[j];
-    const double circ = -zeta[t] * area[t];
-    const double q = FORM == FORM_DOT ? circ / (4 * kPi) : circ / (2 * kPi);
-    src[j] = make_double4(pos.at(t, 0), pos.at(t, 1), pos.at(t, 2), q);
+    const double q = (-zeta[t] * area[t]) / (4 * kPi);
+    const double y0 = pos.at(t, 0), y1 = pos.at(t, 1), y2 = pos.at(t, 2);
+    r.a = make_double4(y0, y1, y2, q);
+    r.b = make_double4(q * y0, q * y1, q * y2, 0.0);
   } else {
-    src[j] = make_double4(0.0, 0.0, 0.0, 0.0);
+    r.a = make_double4(0.0, 0.0, 0.0, 0.0);
+    r.b = make_double4(0.0, 0.0, 0.0, 0.0);
   }
+  src[j] = r;
 }
 
 // FP32C source record: a = {dx, dy, dz, |d|^2} with d = fl32(y - c) the
@@ -139,6 +151,48 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, do
   }
 }
 
+// One workgroup per tile: centre (mean of its sources) and FP64 records;
+// padding records sit 8 units from the centre with q = 0.
+__global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ src, double4* __restrict__ ctr,
+                                                          Pos3 pos, const double* __restrict__ zeta,
+                                                          const double* __restrict__ area,
+                                                          const int* __restrict__ smap, int ns) {
+  __shared__ double red[3][kTile / 64];
+  const int t = blockIdx.x;
+  const int j = t * kTile + threadIdx.x;
+  const bool valid = j < ns;
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, q = 0.0;
+  if (valid) {
+    const int i = smap[j];
+    p0 = pos.at(i, 0);
+    p1 = pos.at(i, 1);
+    p2 = pos.at(i, 2);
+    q = (-zeta[i] * area[i]) / (2 * kPi);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double s0 = wave_sum(p0), s1 = wave_sum(p1), s2 = wave_sum(p2);
+  if (lane == 0) {
+    red[0][wave] = s0;
+    red[1][wave] = s1;
+    red[2][wave] = s2;
+  }
+  __syncthreads();
+  const double inv = 1.0 / min(kTile, ns - t * kTile);
+  double c[3];
+  for (int k = 0; k < 3; ++k) c[k] = ((red[k][0] + red[k][1]) + (red[k][2] + red[k][3])) * inv;
+  Src64 r;
+  if (valid) {
+    const double dx = p0 - c[0], dy = p1 - c[1], dz = p2 - c[2];
+    r.a = make_double4(dx, dy, dz, fma(dx, dx, fma(dy, dy, dz * dz)));
+    r.b = make_double4(q, q * dx, q * dy, q * dz);
+  } else {
+    r.a = make_double4(8.0, 8.0, 8.0, 192.0);
+    r.b = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+  src[j] = r;
+  if (threadIdx.x == 0) ctr[t] = make_double4(c[0], c[1], c[2], 0.0);
+}
+
 // Sum of per-tile q sums in tile order (one workgroup; deterministic).
 __global__ __launch_bounds__(kBlock) void k_qsum(const double4* __restrict__ ctr, int ntiles,
                                                  double* __restrict__ out) {
@@ -202,74 +256,38 @@ __device__ __forceinline__ void store_partials(const PairArgs& a, int by, const 
   }
 }
 
-// One FP64 pair term; `me` drops the target's own source.
-template <int FORM, bool PSI, bool CHECK>
-__device__ __forceinline__ void pair64(const double (&x)[3], const double4 s, bool me, double (&S)[kNC]) {
-  if (FORM == FORM_DOT) {
-    double den = 1.0 - (x[0] * s.x + x[1] * s.y + x[2] * s.z);
-    double q = s.w;
-    if (CHECK && me) {
-      den = 1.0;
-      q = 0.0;
-    }
-    const double f = q * rcp64(den);
-    S[0] = fma(f, s.x, S[0]);
-    S[1] = fma(f, s.y, S[1]);
-    S[2] = fma(f, s.z, S[2]);
-    if (PSI) S[3] = fma(q, log(den), S[3]);
-  } else {
-    const double d0 = s.x - x[0], d1 = s.y - x[1], d2 = s.z - x[2];
-    double r2 = fma(d0, d0, fma(d1, d1, d2 * d2));
-    double q = s.w;
-    if (CHECK && me) {
-      r2 = 1.0;
-      q = 0.0;
-    }
-    const double f = q * rcp64(r2);
-    S[0] = fma(f, d0, S[0]);
-    S[1] = fma(f, d1, S[1]);
-    S[2] = fma(f, d2, S[2]);
-    if (PSI) S[3] = fma(q, log(0.5 * r2), S[3]);
+// Dot-form pair term (biotSavart and greensFn, src/LpmBVEKernels.hpp:21-44): den = 1 - x.y,
+// S += (q y)/den; psi: S3 += q ln(den).  `me` drops the target's own source.
+template <bool PSI, bool CHECK>
+__device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, bool me, double (&S)[kNC]) {
+  double den = 1.0 - (x[0] * s.a.x + x[1] * s.a.y + x[2] * s.a.z);
+  double q = s.a.w;
+  if (CHECK && me) {
+    den = 1.0;
+    q = 0.0;
   }
+  double inv = rcp64(den);
+  if (CHECK && me) inv = 0.0;
+  S[0] = fma(inv, s.b.x, S[0]);
+  S[1] = fma(inv, s.b.y, S[1]);
+  S[2] = fma(inv, s.b.z, S[2]);
+  if (PSI) S[3] = fma(q, log(den), S[3]);
 }
 
-template <int R, int FORM, bool PSI>
-__global__ __launch_bounds__(kBlock) void k_pair_f64(const double4* __restrict__ src, PairArgs a) {
-  double x[R][3];
-  int self[R], idx[R];
-  load_targets<R>(a, blockIdx.x, x, self, idx);
-  double S[R][kNC];
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
-  const int t0 = blockIdx.y * a.tiles_per_split;
-  const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
-  for (int t = t0; t < t1; ++t) {
-    const int base = t * kTile;
-    const double4* ts = src + base;
-    if (!wave_has_self<R>(self, base)) {
-#pragma unroll 4
-      for (int j = 0; j < kTile; ++j) {
-        const double4 s = ts[j];
-#pragma unroll
-        for (int r = 0; r < R; ++r) pair64<FORM, PSI, false>(x[r], s, false, S[r]);
-      }
-    } else {
-      for (int j = 0; j < kTile; ++j) {
-        const double4 s = ts[j];
-#pragma unroll
-        for (int r = 0; r < R; ++r) pair64<FORM, PSI, true>(x[r], s, base + j == self[r], S[r]);
-      }
-    }
-  }
-  store_partials<R>(a, blockIdx.y, idx, S);
-}
-
-// FP64 pair sum with the source tiles staged through LDS (double-buffered,
-// coalesced 8 KB copies per tile), as k_pair_f32c_lds.
-template <int R, int FORM, bool PSI>
-__device__ __forceinline__ void pair_block_f64_lds(const double4* __restrict__ src, const PairArgs& a, int bx,
-                                                   int by, double4 (*sh)[kTile]) {
+// FP64 pair sums of one target block (R targets per lane) over one source
+// split.  LDS: source tiles are staged through LDS (double-buffered, one
+// coalesced 16 KB copy per tile, tile t+1 loaded while tile t is summed);
+// otherwise every lane reads the same record, which the compiler issues as
+// scalar loads.
+//   FORM_DOT: dot_term for every pair.
+//   FORM_CHORD (|y - x|^2/2 form, tile-local records): with D = c - x per
+//     target and tile, tiles holding a target's own source form d = D + dx
+//     explicitly and drop the self pair; other tiles form
+//     r^2 = |D|^2 + |dx|^2 + 2 D.dx and sum F = q/r^2, G = q dx/r^2, S += D F + G
+//     (12 FP64 operations per pair instead of 14).
+template <int R, int FORM, bool PSI, bool LDS>
+__device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, const double4* __restrict__ ctr,
+                                               const PairArgs& a, int bx, int by, Src64 (*sh)[kTile]) {
   double x[R][3];
   int self[R], idx[R];
   load_targets<R>(a, bx, x, self, idx);
@@ -279,36 +297,112 @@ __device__ __forceinline__ void pair_block_f64_lds(const double4* __restrict__ s
     for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
   const int t0 = by * a.tiles_per_split;
   const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
-  if (t0 < t1) sh[0][threadIdx.x] = src[(long)t0 * kTile + threadIdx.x];
-  __syncthreads();
+  if constexpr (LDS) {
+    if (t0 < t1) sh[0][threadIdx.x] = src[(long)t0 * kTile + threadIdx.x];
+    __syncthreads();
+  }
   for (int t = t0; t < t1; ++t) {
-    const int buf = (t - t0) & 1;
-    if (t + 1 < t1) sh[buf ^ 1][threadIdx.x] = src[(long)(t + 1) * kTile + threadIdx.x];
     const int base = t * kTile;
-    const double4* ts = sh[buf];
-    if (!wave_has_self<R>(self, base)) {
+    const Src64* ts;
+    if constexpr (LDS) {
+      const int buf = (t - t0) & 1;
+      if (t + 1 < t1) sh[buf ^ 1][threadIdx.x] = src[(long)(t + 1) * kTile + threadIdx.x];
+      ts = sh[buf];
+    } else {
+      ts = src + base;
+    }
+    const bool has_self = wave_has_self<R>(self, base);
+    if constexpr (FORM == FORM_DOT) {
+      if (!has_self) {
 #pragma unroll 4
-      for (int j = 0; j < kTile; ++j) {
-        const double4 sj = ts[j];
+        for (int j = 0; j < kTile; ++j) {
+          const Src64 sj = ts[j];
 #pragma unroll
-        for (int r = 0; r < R; ++r) pair64<FORM, PSI, false>(x[r], sj, false, S[r]);
+          for (int r = 0; r < R; ++r) dot_term<PSI, false>(x[r], sj, false, S[r]);
+        }
+      } else {
+        for (int j = 0; j < kTile; ++j) {
+          const Src64 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) dot_term<PSI, true>(x[r], sj, base + j == self[r], S[r]);
+        }
       }
     } else {
-      for (int j = 0; j < kTile; ++j) {
-        const double4 sj = ts[j];
+      const double4 c = ctr[t];
+      double D[R][3];
 #pragma unroll
-        for (int r = 0; r < R; ++r) pair64<FORM, PSI, true>(x[r], sj, base + j == self[r], S[r]);
+      for (int r = 0; r < R; ++r) {
+        D[r][0] = c.x - x[r][0];
+        D[r][1] = c.y - x[r][1];
+        D[r][2] = c.z - x[r][2];
+      }
+      if (has_self) {
+        for (int j = 0; j < kTile; ++j) {
+          const Src64 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const double e0 = D[r][0] + sj.a.x, e1 = D[r][1] + sj.a.y, e2 = D[r][2] + sj.a.z;
+            double r2 = fma(e0, e0, fma(e1, e1, e2 * e2));
+            double q = sj.b.x;
+            if (base + j == self[r]) {
+              r2 = 1.0;
+              q = 0.0;
+            }
+            const double f = q * rcp64(r2);
+            S[r][0] = fma(f, e0, S[r][0]);
+            S[r][1] = fma(f, e1, S[r][1]);
+            S[r][2] = fma(f, e2, S[r][2]);
+            if (PSI) S[r][3] = fma(q, log(0.5 * r2), S[r][3]);
+          }
+        }
+      } else {
+        double D2[R][3], E[R], F[R][kNC + 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          for (int k = 0; k < 3; ++k) D2[r][k] = 2.0 * D[r][k];
+          E[r] = fma(D[r][0], D[r][0], fma(D[r][1], D[r][1], D[r][2] * D[r][2]));
+          for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0;
+        }
+#pragma unroll 2
+        for (int j = 0; j < kTile; ++j) {
+          const Src64 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            double r2 = E[r] + sj.a.w;
+            r2 = fma(D2[r][0], sj.a.x, r2);
+            r2 = fma(D2[r][1], sj.a.y, r2);
+            r2 = fma(D2[r][2], sj.a.z, r2);
+            const double inv = rcp64(r2);
+            F[r][0] = fma(inv, sj.b.x, F[r][0]);
+            F[r][1] = fma(inv, sj.b.y, F[r][1]);
+            F[r][2] = fma(inv, sj.b.z, F[r][2]);
+            F[r][3] = fma(inv, sj.b.w, F[r][3]);
+            if (PSI) F[r][4] = fma(sj.b.x, log(0.5 * r2), F[r][4]);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          S[r][0] += fma(D[r][0], F[r][0], F[r][1]);
+          S[r][1] += fma(D[r][1], F[r][0], F[r][2]);
+          S[r][2] += fma(D[r][2], F[r][0], F[r][3]);
+          S[r][3] += F[r][4];
+        }
       }
     }
-    __syncthreads();
+    if constexpr (LDS) __syncthreads();
   }
   store_partials<R>(a, by, idx, S);
 }
 
-template <int R, int FORM, bool PSI>
-__global__ __launch_bounds__(kBlock) void k_pair_f64_lds(const double4* __restrict__ src, PairArgs a) {
-  __shared__ double4 sh[2][kTile];
-  pair_block_f64_lds<R, FORM, PSI>(src, a, blockIdx.x, blockIdx.y, sh);
+template <int R, int FORM, bool PSI, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_pair_f64(const Src64* __restrict__ src, const double4* __restrict__ ctr,
+                                                     PairArgs a) {
+  if constexpr (LDS) {
+    __shared__ Src64 sh[2][kTile];
+    pair_block_f64<R, FORM, PSI, true>(src, ctr, a, blockIdx.x, blockIdx.y, sh);
+  } else {
+    pair_block_f64<R, FORM, PSI, false>(src, ctr, a, blockIdx.x, blockIdx.y, nullptr);
+  }
 }
 
 // Near form: d = D + dx formed explicitly, f = q / |d|^2, S += f d.
@@ -545,7 +639,7 @@ struct EnsMember {
   const Src32* src32;
   const double4* ctr;
   const float* trad;
-  const double4* src64;
+  const Src64* src64;
 };
 
 // One work item: target block bx and source split by of member m.  Items
@@ -587,14 +681,14 @@ template <int R, int FORM>
 __global__ __launch_bounds__(kBlock) void k_pair_f64_ens(const EnsMember* __restrict__ mem,
                                                          const EnsItem* __restrict__ items, int nitems,
                                                          int* counter) {
-  __shared__ double4 sh[2][kTile];
+  __shared__ Src64 sh[2][kTile];
   __shared__ int slot;
   for (;;) {
     const int it = next_item(counter, &slot);
     if (it >= nitems) break;
     const EnsItem w = items[it];
     const EnsMember& m = mem[w.member];
-    pair_block_f64_lds<R, FORM, false>(m.src64, m.a, w.bx, w.by, sh);
+    pair_block_f64<R, FORM, false, true>(m.src64, m.ctr, m.a, w.bx, w.by, sh);
   }
 }
 
@@ -835,7 +929,7 @@ Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
 
 // Device buffers of one evaluation: sources, partials, FP32C extras.
 struct EvalBuffers {
-  double4* src64 = nullptr;
+  Src64* src64 = nullptr;  // dot-form or tile-local FP64 records
   Src32* src32 = nullptr;
   double4* ctr = nullptr;
   float* trad = nullptr;
@@ -865,6 +959,21 @@ struct EvalRequest {
   Shape shape;
 };
 
+template <int R, bool LDS>
+void launch_pair_f64(const EvalRequest& q, const EvalBuffers& b, const PairArgs& pa, dim3 grid, hipStream_t st) {
+  if (q.form == FORM_DOT) {
+    if (q.want_psi)
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_DOT, true, LDS>), grid, dim3(kBlock), 0, st, b.src64, b.ctr, pa);
+    else
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_DOT, false, LDS>), grid, dim3(kBlock), 0, st, b.src64, b.ctr, pa);
+  } else {
+    if (q.want_psi)
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_CHORD, true, LDS>), grid, dim3(kBlock), 0, st, b.src64, b.ctr, pa);
+    else
+      hipLaunchKernelGGL((k_pair_f64<R, FORM_CHORD, false, LDS>), grid, dim3(kBlock), 0, st, b.src64, b.ctr, pa);
+  }
+}
+
 template <int R>
 void launch_pair(const EvalRequest& q, const EvalBuffers& b, const PairArgs& pa, dim3 grid, hipStream_t st) {
   if (q.prec == LPM_PREC_FP32C) {
@@ -879,27 +988,9 @@ void launch_pair(const EvalRequest& q, const EvalBuffers& b, const PairArgs& pa,
       hipLaunchKernelGGL((k_pair_f32c<R, false>), grid, dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa);
     }
   } else if (q.shape.lds) {
-    if (q.form == FORM_DOT) {
-      if (q.want_psi)
-        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_DOT, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
-      else
-        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_DOT, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
-    } else {
-      if (q.want_psi)
-        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_CHORD, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
-      else
-        hipLaunchKernelGGL((k_pair_f64_lds<R, FORM_CHORD, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
-    }
-  } else if (q.form == FORM_DOT) {
-    if (q.want_psi)
-      hipLaunchKernelGGL((k_pair_f64<R, FORM_DOT, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
-    else
-      hipLaunchKernelGGL((k_pair_f64<R, FORM_DOT, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
+    launch_pair_f64<R, true>(q, b, pa, grid, st);
   } else {
-    if (q.want_psi)
-      hipLaunchKernelGGL((k_pair_f64<R, FORM_CHORD, true>), grid, dim3(kBlock), 0, st, b.src64, pa);
-    else
-      hipLaunchKernelGGL((k_pair_f64<R, FORM_CHORD, false>), grid, dim3(kBlock), 0, st, b.src64, pa);
+    launch_pair_f64<R, false>(q, b, pa, grid, st);
   }
 }
 
@@ -925,11 +1016,11 @@ int enqueue_prep(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
                        q.zeta, q.area, q.smap, q.ns);
     if (q.want_psi) hipLaunchKernelGGL(k_qsum, dim3(1), dim3(kBlock), 0, st, b.ctr, sh.ntiles, b.qtot);
   } else if (q.form == FORM_DOT) {
-    hipLaunchKernelGGL((k_prep_f64<FORM_DOT>), dim3(cdiv(ns_pad, kBlock)), dim3(kBlock), 0, st, b.src64, q.srcpos,
-                       q.zeta, q.area, q.smap, q.ns, ns_pad);
+    hipLaunchKernelGGL(k_prep_f64_dot, dim3(cdiv(ns_pad, kBlock)), dim3(kBlock), 0, st, b.src64, q.srcpos, q.zeta,
+                       q.area, q.smap, q.ns, ns_pad);
   } else {
-    hipLaunchKernelGGL((k_prep_f64<FORM_CHORD>), dim3(cdiv(ns_pad, kBlock)), dim3(kBlock), 0, st, b.src64,
-                       q.srcpos, q.zeta, q.area, q.smap, q.ns, ns_pad);
+    hipLaunchKernelGGL(k_prep_f64_tiles, dim3(sh.ntiles), dim3(kTile), 0, st, b.src64, b.ctr, q.srcpos, q.zeta,
+                       q.area, q.smap, q.ns);
   }
   LPM_HIP(hipGetLastError());
   return LPM_OK;
@@ -990,7 +1081,7 @@ struct BufferPlan {
 BufferPlan plan_buffers(int nt, const Shape& sh) {
   const size_t ns_pad = (size_t)sh.ntiles * kTile;
   BufferPlan p;
-  p.src64 = ns_pad * sizeof(double4);
+  p.src64 = ns_pad * sizeof(Src64);
   p.src32 = ns_pad * sizeof(Src32);
   p.ctr = (size_t)sh.ntiles * sizeof(double4) + 256;
   p.trad = (size_t)sh.ntiles * sizeof(float) + 256;
@@ -1054,7 +1145,7 @@ int functor_eval(double* u, double* psi, const double* tx, int64_t nt, const dou
     return r;
   };
   EvalBuffers b;
-  b.src64 = (double4*)carve(bp.src64);
+  b.src64 = (Src64*)carve(bp.src64);
   b.part = (double*)carve(bp.part);
   int* smap = (int*)carve(sizeof(int) * (nf + 1));
   int* self = (int*)carve(sizeof(int) * (nf + 1));
@@ -1814,6 +1905,7 @@ int ens_eval(lpm_bve_ensemble* e, int which, int prec, int form, int pass) {
   for (size_t m = 0; m < e->members.size(); ++m) {
     lpm_bve_ctx* c = e->members[m];
     if ((rc = ctx_request(c, which ? c->x : c->wx, which ? c->zeta : c->wzeta, prec, form, false, qs[m]))) return rc;
+    qs[m].shape.lds = true;  // the persistent kernels stage sources through LDS
     if ((rc = enqueue_prep(qs[m], c->eb, e->stream))) return rc;
   }
   const EnsMember* mem = which ? e->d_mem_x : e->d_mem_w;
