@@ -11,8 +11,10 @@ solve, which is the next step's stage 1).
 
 value  = FP32C pair interactions per second over the K timed steps, whole
          job, state resident in HBM, steps replayed from a captured hipGraph.
-Also reported: the FP64 path's rate on the same step, the FP32C/FP64
-speed-up, FP32C error against FP64 for one evaluation, the pair kernel's
+Also reported: the FP64 path's rate over 10 steps from the same initial
+state, the FP32C/FP64 speed-up, FP32C error against FP64 (positions after
+those 10 steps; velocity from the step kernel and stream function at the
+final FP64 state), the FP64 reference scheme's rate, the pair kernel's
 roofline position (HIP events around the kernel on its own stream), and the
 CPU baseline: the oracle's reference-structured loop on a bounded sample.
 
@@ -35,6 +37,9 @@ sys.path.insert(0, REPO)
 
 FLOPS_PER_PAIR = 15
 FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+# HBM bytes per level-8 FP32C pair launch: FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc
+# runs, each scaled by the same counter's reading for a device copy of known size
+TRAFFIC_BYTES_PER_LAUNCH = 10.06e9 + 126.1e6
 DT = 0.005
 OMEGA = 2 * np.pi
 
@@ -189,7 +194,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--level", type=int, default=8)
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--compare-steps", type=int, default=10,
+                    help="FP64 and FP32C steps from the same state: FP64 rate and FP32C error (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -254,38 +260,65 @@ def main() -> int:
     achieved = FLOPS_PER_PAIR * pairs_eval / (kernel_ms * 1e-3) / 1e12
     log(f"[bench] pair kernel: {nlaunch} launches, {kernel_ms:.2f} ms each, {achieved:.2f} TFLOP/s")
 
-    # FP64 on the same step
-    rate64 = None
-    if args.fp64_steps > 0:
-        ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=1)  # warm (capture + entry velocity)
-        ctx.synchronize()
+    # FP64 and FP32C from the same initial state over --compare-steps steps (the FP64 run is
+    # also the FP64 rate), then FP32C error against FP64: positions after the steps, velocity
+    # from the step kernel and stream function at the final FP64 state
+    rate64 = rate64_ref = err = None
+    nsteps_cmp = args.compare_steps
+    if nsteps_cmp > 0:
+        c64 = L.Context(0)
+        c64.set_state(mesh.vx, vz, fx, fz, farea, np.zeros(nl, np.uint8))
+        c64.solve(L.PREC_FP64, L.SCHEME_LAGR)  # entry velocity (stage 1 of the first step)
+        c64.synchronize()
         t2 = time.perf_counter()
-        ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=args.fp64_steps)
-        ctx.synchronize()
+        c64.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP64, nsteps=nsteps_cmp)
+        c64.synchronize()
         t_fp64 = time.perf_counter() - t2
-        rate64 = pairs_step * args.fp64_steps / t_fp64
-        log(f"[bench] FP64: {1e3 * t_fp64 / args.fp64_steps:.1f} ms/step, {rate64:.4e} pairs/s")
-
-    # FP32C error against FP64 for one evaluation at the current state
-    err = None
-    if not args.no_accuracy:
-        ctx.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
-        a = ctx.get_state()
-        ctx.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
-        b = ctx.get_state()
-        u64 = np.concatenate([a["vu"], a["fu"]])
-        u32 = np.concatenate([b["vu"], b["fu"]])
-        p64 = np.concatenate([a["vpsi"], a["fpsi"]])
-        p32 = np.concatenate([b["vpsi"], b["fpsi"]])
-        du = np.linalg.norm(u32 - u64, axis=1)
-        umax = np.linalg.norm(u64, axis=1).max()
-        err = {
-            "velocity_max_rel": float(du.max() / umax),
-            "velocity_rms_rel": float(np.sqrt(np.mean(du ** 2)) / np.sqrt(np.mean(np.sum(u64 ** 2, axis=1)))),
-            "psi_max_rel": float(np.abs(p32 - p64).max() / np.abs(p64).max()),
-            "psi_rms_rel": float(np.sqrt(np.mean((p32 - p64) ** 2) / np.mean(p64 ** 2))),
-        }
-        log(f"[bench] FP32C vs FP64: {err}")
+        rate64 = pairs_step * nsteps_cmp / t_fp64
+        log(f"[bench] FP64: {1e3 * t_fp64 / nsteps_cmp:.1f} ms/step, {rate64:.4e} pairs/s ({nsteps_cmp} steps)")
+        if not args.no_accuracy:
+            c32 = L.Context(0)
+            c32.set_state(mesh.vx, vz, fx, fz, farea, np.zeros(nl, np.uint8))
+            c32.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=nsteps_cmp)
+            a, b = c64.get_state(), c32.get_state()
+            x64 = np.concatenate([a["vx"], a["fx"]])
+            x32 = np.concatenate([b["vx"], b["fx"]])
+            dx = np.linalg.norm(x32 - x64, axis=1)  # |x| = 1: absolute = relative
+            # one evaluation at the FP64 state in both precisions
+            c64.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
+            a = c64.get_state()
+            c32.set_state(a["vx"], a["vzeta"], a["fx"], a["fzeta"], farea, np.zeros(nl, np.uint8))
+            c32.solve(L.PREC_FP32C, L.SCHEME_LAGR)  # velocity with the kernel the timed steps use
+            bv = c32.get_state()
+            c32.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
+            bp = c32.get_state()
+            c32.close()
+            u64 = np.concatenate([a["vu"], a["fu"]])
+            u32 = np.concatenate([bv["vu"], bv["fu"]])
+            p64 = np.concatenate([a["vpsi"], a["fpsi"]])
+            p32 = np.concatenate([bp["vpsi"], bp["fpsi"]])
+            du = np.linalg.norm(u32 - u64, axis=1)
+            umax = np.linalg.norm(u64, axis=1).max()
+            err = {
+                "steps": nsteps_cmp,
+                "positions_max_rel": float(dx.max()),
+                "positions_rms_rel": float(np.sqrt(np.mean(dx ** 2))),
+                "velocity_max_rel": float(du.max() / umax),
+                "velocity_rms_rel": float(np.sqrt(np.mean(du ** 2)) / np.sqrt(np.mean(np.sum(u64 ** 2, axis=1)))),
+                "psi_max_rel": float(np.abs(p32 - p64).max() / np.abs(p64).max()),
+                "psi_rms_rel": float(np.sqrt(np.mean((p32 - p64) ** 2) / np.mean(p64 ** 2))),
+            }
+            log(f"[bench] FP32C vs FP64: {err}")
+        # the reference scheme (literal 1 - x.y, no renormalisation) from the FP64 state
+        c64.advance(DT, OMEGA, L.SCHEME_REF, L.PREC_FP64, nsteps=1)  # warm (capture + entry velocity)
+        c64.synchronize()
+        t3 = time.perf_counter()
+        c64.advance(DT, OMEGA, L.SCHEME_REF, L.PREC_FP64, nsteps=1)
+        c64.synchronize()
+        t_ref = time.perf_counter() - t3
+        rate64_ref = pairs_step / t_ref
+        log(f"[bench] FP64 reference scheme: {1e3 * t_ref:.1f} ms/step, {rate64_ref:.4e} pairs/s")
+        c64.close()
     ctx.close()
 
     cpu = None
@@ -322,6 +355,7 @@ def main() -> int:
         },
         "rk4_steps_per_s": 1e3 / ms_step,
         "fp64_pairs_per_s": rate64,
+        "fp64_reference_scheme_pairs_per_s": rate64_ref,
         "fp32c_vs_fp64_speedup": rate / rate64 if rate64 else None,
         "fp32c_error_vs_fp64": err,
         "roofline": {
@@ -330,7 +364,8 @@ def main() -> int:
             "peak": FP32_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if args.level == 8 else None,
+            "traffic_source": "profiles/k_pair_f32c_lds_level8_counters.md" if args.level == 8 else None,
             "kernel": "k_pair_f32c",
             "kernel_ms": kernel_ms,
             "flops_per_pair": FLOPS_PER_PAIR,

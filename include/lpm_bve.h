@@ -161,7 +161,7 @@ int lpm_bve_synchronize(lpm_bve_ctx* ctx);
 /* Sizes of the resident problem: {nv, nf, nsources, ntargets}. */
 int lpm_bve_sizes(lpm_bve_ctx* ctx, int64_t* sizes4);
 
-/* Kernel launch shape: targets per lane (1, 2 or 4; 0 = automatic) and
+/* Kernel launch shape: targets per lane (1 to 4; 0 = automatic) and
  * source splits (0 = automatic).  Applies to later solves and steps. */
 int lpm_bve_set_launch(lpm_bve_ctx* ctx, int targets_per_lane, int source_splits);
 

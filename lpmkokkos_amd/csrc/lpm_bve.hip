@@ -440,6 +440,124 @@ __device__ __forceinline__ void pair32_far(const float (&D2)[3], float E, const 
   if (PSI) F[4] = fmaf(s.b.x, __builtin_amdgcn_logf(r2), F[4]);
 }
 
+// Far form of the step kernel (velocity only).  On the unit sphere
+// |y - x|^2 / 2 = 1 - x.y; in tile-local coordinates y = c + dx this is
+//   den = (1 - x.c) - x.dx,
+// where A = 1 - x.c is formed per target and tile in FP64 (no cancellation
+// there) and rounded once, and x.dx is small: on a tile beyond 4 tile radii
+// the cancellation in den is at most ~4, as for |D|^2 + |dx|^2 + 2 D.dx.
+// 3 FMAs per pair; F = sum q/den and G = sum q dx/den give the tile's
+// sum of q d/|d|^2 as (D F + G) / 2.
+__device__ __forceinline__ float far_den(const float (&NX)[3], float A, const Src32& s) {
+  float v = fmaf(NX[0], s.a.x, A);
+  v = fmaf(NX[1], s.a.y, v);
+  return fmaf(NX[2], s.a.z, v);
+}
+
+__device__ __forceinline__ void far_acc(float inv, const Src32& s, float (&F)[kNC + 1]) {
+  F[0] = fmaf(inv, s.b.x, F[0]);
+  F[1] = fmaf(inv, s.b.y, F[1]);
+  F[2] = fmaf(inv, s.b.z, F[2]);
+  F[3] = fmaf(inv, s.b.w, F[3]);
+}
+
+// Four pairs sharing one reciprocal: with a, b, c, d the four denominators,
+// p = 1/(ab cd), 1/a = b (cd p), 1/b = a (cd p), 1/c = d (ab p),
+// 1/d = c (ab p).  9 multiplies and one v_rcp_f32 replace four v_rcp_f32,
+// which issue at a quarter of the FMA rate and do not overlap it.  Far-form
+// denominators lie in [((near_factor - 1) r_tile)^2 / 2, 126] (126 for
+// padding records), so the product of four stays in the FP32 normal range;
+// each reciprocal carries about 5 roundings instead of 1.
+__device__ __forceinline__ void pair32_far4(const float (&NX)[3], float A, const Src32& s0, const Src32& s1,
+                                            const Src32& s2, const Src32& s3, float (&F)[kNC + 1]) {
+  const float a = far_den(NX, A, s0), b = far_den(NX, A, s1);
+  const float c = far_den(NX, A, s2), d = far_den(NX, A, s3);
+  const float ab = a * b, cd = c * d;
+  const float p = __builtin_amdgcn_rcpf(ab * cd);
+  const float iab = cd * p, icd = ab * p;
+  far_acc(b * iab, s0, F);
+  far_acc(a * iab, s1, F);
+  far_acc(d * icd, s2, F);
+  far_acc(c * icd, s3, F);
+}
+
+// The same for two targets (A, B) and two sources.
+__device__ __forceinline__ void pair32_far22(const float (&NXA)[3], float AA, const float (&NXB)[3], float AB,
+                                             const Src32& s0, const Src32& s1, float (&FA)[kNC + 1],
+                                             float (&FB)[kNC + 1]) {
+  const float a = far_den(NXA, AA, s0), b = far_den(NXA, AA, s1);
+  const float c = far_den(NXB, AB, s0), d = far_den(NXB, AB, s1);
+  const float ab = a * b, cd = c * d;
+  const float p = __builtin_amdgcn_rcpf(ab * cd);
+  const float iab = cd * p, icd = ab * p;
+  far_acc(b * iab, s0, FA);
+  far_acc(a * iab, s1, FA);
+  far_acc(d * icd, s0, FB);
+  far_acc(c * icd, s1, FB);
+}
+
+// The far form reads 28 of a record's 32 bytes; keeping the unused |dx|^2
+// live makes each half a single 16-byte LDS read instead of 12 + 16 bytes.
+__device__ __forceinline__ Src32 load_src(const Src32* p) {
+  Src32 r = *p;
+  asm volatile("" ::"v"(r.a.w));
+  return r;
+}
+
+// Far-form sums of one tile for R targets (velocity only), one reciprocal per
+// four pairs: target pairs x source pairs, and four sources for the last
+// target when R is odd.
+template <int R>
+__device__ __forceinline__ void far_tile(const Src32* ts, const float (&NX)[R][3], const float (&A)[R],
+                                         float (&F)[R][kNC + 1]) {
+  if constexpr (R == 1) {
+    for (int j = 0; j < kTile; j += 4) {
+      const Src32 s0 = load_src(ts + j), s1 = load_src(ts + j + 1), s2 = load_src(ts + j + 2),
+                  s3 = load_src(ts + j + 3);
+      pair32_far4(NX[0], A[0], s0, s1, s2, s3, F[0]);
+    }
+  } else if constexpr (R % 2 == 0) {
+    for (int j = 0; j < kTile; j += 2) {
+      const Src32 s0 = load_src(ts + j), s1 = load_src(ts + j + 1);
+#pragma unroll
+      for (int r = 0; r < R; r += 2) pair32_far22(NX[r], A[r], NX[r + 1], A[r + 1], s0, s1, F[r], F[r + 1]);
+    }
+  } else {
+    for (int j = 0; j < kTile; j += 4) {
+      const Src32 s0 = load_src(ts + j), s1 = load_src(ts + j + 1), s2 = load_src(ts + j + 2),
+                  s3 = load_src(ts + j + 3);
+#pragma unroll
+      for (int r = 0; r + 1 < R; r += 2) {
+        pair32_far22(NX[r], A[r], NX[r + 1], A[r + 1], s0, s1, F[r], F[r + 1]);
+        pair32_far22(NX[r], A[r], NX[r + 1], A[r + 1], s2, s3, F[r], F[r + 1]);
+      }
+      pair32_far4(NX[R - 1], A[R - 1], s0, s1, s2, s3, F[R - 1]);
+    }
+  }
+}
+
+// A tile's far-form contribution T for R targets (velocity-only kernels):
+// A = 1 - x.c in FP64, the sums, then (D F + G) / 2.
+template <int R>
+__device__ __forceinline__ void far_tile_sum(const Src32* ts, const double (&x)[R][3], const double4& c,
+                                             const float (&NX)[R][3], const float (&D)[R][3],
+                                             float (&T)[R][kNC]) {
+  float A[R], F[R][kNC + 1];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    A[r] = (float)(1.0 - (x[r][0] * c.x + x[r][1] * c.y + x[r][2] * c.z));
+    for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0f;
+  }
+  far_tile<R>(ts, NX, A, F);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    T[r][0] = 0.5f * fmaf(D[r][0], F[r][0], F[r][1]);
+    T[r][1] = 0.5f * fmaf(D[r][1], F[r][0], F[r][2]);
+    T[r][2] = 0.5f * fmaf(D[r][2], F[r][0], F[r][3]);
+    T[r][3] = 0.0f;
+  }
+}
+
 // Tiles whose centre lies within `near_factor` tile radii of a target of
 // the wave use the near form, and so does the tile holding a target's own
 // source; the far form's cancellation in |d|^2 is bounded by
@@ -453,6 +571,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
   double x[R][3];
   int self[R], idx[R];
   load_targets<R>(a, blockIdx.x, x, self, idx);
+  float NX[R][3];  // -x in FP32 (far form of the velocity-only kernels)
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    for (int k = 0; k < 3; ++k) NX[r][k] = -(float)x[r][k];
   double S[R][kNC];
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -494,29 +616,17 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
           for (int r = 0; r < R; ++r) pair32_near<PSI, true>(D[r], sj, base + j == self[r], T[r]);
         }
       }
-    } else {
+    } else if constexpr (PSI) {
       float D2[R][3], F[R][kNC + 1];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         for (int k = 0; k < 3; ++k) D2[r][k] = 2.0f * D[r][k];
         for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0f;
       }
-      for (int j = 0; j < kTile; j += 4) {
+      for (int j = 0; j < kTile; ++j) {
+        const Src32 sj = ts[j];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const Src32 sj = ts[j + u];
-#pragma unroll
-          for (int r = 0; r < R; ++r) pair32_far<PSI>(D2[r], E[r], sj, F[r]);
-        }
-        // Spread the reciprocals through the iteration: the transcendental
-        // unit runs beside the VALU, but a run of back-to-back v_rcp stalls
-        // the issuing wave.  Pattern per pair: 4 VALU, 1 TRANS, 4 VALU.
-#pragma unroll
-        for (int k = 0; k < 4 * R; ++k) {
-          __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
-          __builtin_amdgcn_sched_group_barrier(0x400, PSI ? 2 : 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
-        }
+        for (int r = 0; r < R; ++r) pair32_far<PSI>(D2[r], E[r], sj, F[r]);
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -525,6 +635,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
         T[r][2] = fmaf(D[r][2], F[r][0], F[r][3]);
         T[r][3] = F[r][4];
       }
+    } else {
+      far_tile_sum<R>(ts, x, c, NX, D, T);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -538,13 +650,22 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
 // it starts tile t, and waves read records with uniform-address LDS loads
 // into VGPRs (LDS returns in order, so the compiler prefetches the next
 // records while the current ones are in use).
+// With xs (R >= 3), each lane parks its targets' FP64 positions in its own
+// LDS slots ([3R][kBlock], 8 KB per target per lane) and reloads them once
+// per tile, which keeps the register count at four waves per SIMD.
 template <int R, bool PSI>
 __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ src, const double4* __restrict__ ctr,
                                                     const float* __restrict__ trad, const PairArgs& a, int bx,
-                                                    int by, Src32 (*sh)[kTile]) {
+                                                    int by, Src32 (*sh)[kTile], double* xs) {
+  constexpr bool kXL = R >= 3;
   double x[R][3];
   int self[R], idx[R];
   load_targets<R>(a, bx, x, self, idx);
+  if constexpr (kXL) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      for (int k = 0; k < 3; ++k) xs[(3 * r + k) * kBlock + threadIdx.x] = x[r][k];
+  }
   double S[R][kNC];
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -561,7 +682,12 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
     const int base = t * kTile;
     const double4 c = ctr[t];
     const float rn = a.near_factor * trad[t];
-    float D[R][3], E[R];
+    if constexpr (kXL) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        for (int k = 0; k < 3; ++k) x[r][k] = xs[(3 * r + k) * kBlock + threadIdx.x];
+    }
+    float D[R][3], E[R], NX[R][3];  // NX = -x in FP32 (far form of the velocity-only kernels)
     int near = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -570,6 +696,7 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
       D[r][2] = (float)(c.z - x[r][2]);
       E[r] = fmaf(D[r][0], D[r][0], fmaf(D[r][1], D[r][1], D[r][2] * D[r][2]));
       near |= E[r] < rn * rn;
+      for (int k = 0; k < 3; ++k) NX[r][k] = -(float)x[r][k];
     }
     const Src32* ts = sh[buf];
     float T[R][kNC];
@@ -592,7 +719,7 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
           for (int r = 0; r < R; ++r) pair32_near<PSI, true>(D[r], sj, base + j == self[r], T[r]);
         }
       }
-    } else {
+    } else if constexpr (PSI) {
       float D2[R][3], F[R][kNC + 1];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -612,6 +739,8 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
         T[r][2] = fmaf(D[r][2], F[r][0], F[r][3]);
         T[r][3] = F[r][4];
       }
+    } else {
+      far_tile_sum<R>(ts, x, c, NX, D, T);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -626,7 +755,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c_lds(const Src32* __restric
                                                           const double4* __restrict__ ctr,
                                                           const float* __restrict__ trad, PairArgs a) {
   __shared__ Src32 sh[2][kTile];
-  pair_block_f32c_lds<R, PSI>(src, ctr, trad, a, blockIdx.x, blockIdx.y, sh);
+  if constexpr (R >= 3) {
+    __shared__ double xs[3 * R * kBlock];
+    pair_block_f32c_lds<R, PSI>(src, ctr, trad, a, blockIdx.x, blockIdx.y, sh, xs);
+  } else {
+    pair_block_f32c_lds<R, PSI>(src, ctr, trad, a, blockIdx.x, blockIdx.y, sh, nullptr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -667,13 +801,14 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c_ens(const EnsMember* __res
                                                           const EnsItem* __restrict__ items, int nitems,
                                                           int* counter) {
   __shared__ Src32 sh[2][kTile];
+  __shared__ double xs[R >= 3 ? 3 * R * kBlock : 1];
   __shared__ int slot;
   for (;;) {
     const int it = next_item(counter, &slot);
     if (it >= nitems) break;
     const EnsItem w = items[it];
     const EnsMember& m = mem[w.member];
-    pair_block_f32c_lds<R, false>(m.src32, m.ctr, m.trad, m.a, w.bx, w.by, sh);
+    pair_block_f32c_lds<R, false>(m.src32, m.ctr, m.trad, m.a, w.bx, w.by, sh, xs);
   }
 }
 
@@ -913,7 +1048,10 @@ Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
     }
   }
   Shape sh;
-  sh.R = (R_req == 1 || R_req == 2 || R_req == 4) ? R_req : 2;
+  // default without a rule: four targets per lane on large problems (half the
+  // LDS reads per pair of two; 1,920 workgroups x 2 source splits at level 8),
+  // two below a million targets, where four would leave too few workgroups
+  sh.R = (R_req >= 1 && R_req <= 4) ? R_req : (nt >= (1 << 20) ? 4 : 2);
   sh.ntiles = std::max(1, cdiv(ns, kTile));
   const int wg_x = cdiv(nt, (long)kBlock * sh.R);
   int splits = splits_req;
@@ -1033,6 +1171,7 @@ int enqueue_pair(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
   if (ev0) LPM_HIP(hipEventRecord(ev0, st));
   switch (q.shape.R) {
     case 1: launch_pair<1>(q, b, pa, grid, st); break;
+    case 3: launch_pair<3>(q, b, pa, grid, st); break;
     case 4: launch_pair<4>(q, b, pa, grid, st); break;
     default: launch_pair<2>(q, b, pa, grid, st); break;
   }
@@ -1683,8 +1822,8 @@ int lpm_bve_sizes(lpm_bve_ctx* c, int64_t* s) {
 
 int lpm_bve_set_launch(lpm_bve_ctx* c, int targets_per_lane, int source_splits) {
   if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_launch: null context");
-  if (targets_per_lane != 0 && targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4)
-    return fail(LPM_ERR_ARG, "lpm_bve_set_launch: targets_per_lane must be 0 (automatic), 1, 2 or 4");
+  if (targets_per_lane < 0 || targets_per_lane > 4)
+    return fail(LPM_ERR_ARG, "lpm_bve_set_launch: targets_per_lane must be 0 (automatic) or 1..4");
   if (source_splits < 0) return fail(LPM_ERR_ARG, "lpm_bve_set_launch: source_splits < 0");
   c->R_req = targets_per_lane;
   c->splits_req = source_splits;
@@ -1890,7 +2029,10 @@ int ens_grid(lpm_bve_ensemble* e, int R, int prec, int form, int nitems) {
   if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return 256;
   int per_cu = e->blocks_per_cu;
   if (per_cu <= 0) {
-    const void* k = R == 1 ? ens_kernel<1>(prec, form) : R == 4 ? ens_kernel<4>(prec, form) : ens_kernel<2>(prec, form);
+    const void* k = R == 1   ? ens_kernel<1>(prec, form)
+                    : R == 3 ? ens_kernel<3>(prec, form)
+                    : R == 4 ? ens_kernel<4>(prec, form)
+                             : ens_kernel<2>(prec, form);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
   }
   return std::max(1, std::min(nitems, per_cu * prop.multiProcessorCount));
@@ -1914,6 +2056,7 @@ int ens_eval(lpm_bve_ensemble* e, int which, int prec, int form, int pass) {
     const int R = e->plan_R[k];
     const int grid = e->plan_grid[k];
     if (R == 1) rc = ens_launch_R<1>(e, (int)k, mem, counter, prec, form, grid);
+    else if (R == 3) rc = ens_launch_R<3>(e, (int)k, mem, counter, prec, form, grid);
     else if (R == 4) rc = ens_launch_R<4>(e, (int)k, mem, counter, prec, form, grid);
     else rc = ens_launch_R<2>(e, (int)k, mem, counter, prec, form, grid);
     if (rc) return rc;
@@ -2069,7 +2212,7 @@ int lpm_ens_synchronize(lpm_bve_ensemble* e) {
 // launch-shape autotuning, keyed on the number of targets N
 // ---------------------------------------------------------------------------
 int lpm_shape_rule_set(int64_t ntargets, int targets_per_lane, int source_splits) {
-  if (ntargets < 1 || (targets_per_lane != 1 && targets_per_lane != 2 && targets_per_lane != 4) ||
+  if (ntargets < 1 || targets_per_lane < 1 || targets_per_lane > 4 ||
       source_splits < 0)
     return fail(LPM_ERR_ARG, "lpm_shape_rule_set: bad arguments");
   std::lock_guard<std::mutex> g(g_rule_mu);
@@ -2103,7 +2246,7 @@ int lpm_bve_autotune(lpm_bve_ctx* c, int prec, int scheme, int reps, int* best_R
   const bool saved_timing = c->timing;
   const int ntiles = std::max(1, cdiv(c->ns, kTile));
   std::vector<std::pair<int, int>> cands;
-  for (int R : {1, 2, 4}) {
+  for (int R : {1, 2, 3, 4}) {
     const Shape base = choose_shape(c->nt, std::max(c->ns, 1), R, 0);
     std::vector<int> sp = {base.splits, 1, 2, 4, 8, 16};
     std::sort(sp.begin(), sp.end());

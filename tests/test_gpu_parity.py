@@ -153,6 +153,34 @@ def test_fp32c_vs_fp64(omesh, level):
     c.close()
 
 
+@pytest.mark.parametrize("level", [6, 7])
+def test_fp32c_step_kernel_vs_fp64(omesh, level):
+    """The velocity-only FP32C kernel (the one inside RK4 steps; its far form
+    shares one reciprocal among four pairs) against FP64 for every launch shape
+    and both source stagings: within 1e-6 normwise, and within 1e-6 of the
+    FP32C kernel that also sums the stream function."""
+    m = omesh(level)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=1, nvort=32, sigma=0.1, rh=(4, 0.1, 0.1))
+    fz = fz[m.fmask == 0]
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR)
+    a = c.get_state()
+    c.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=True)
+    p = c.get_state()
+    for lds in (True, False):
+        for R in (1, 2, 3, 4):
+            c.set_source_staging(lds)
+            c.set_launch(R, 0)
+            c.solve(L.PREC_FP32C, L.SCHEME_LAGR)
+            b = c.get_state()
+            eu = max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"]))
+            ep = max(normwise(b["vu"], p["vu"]), normwise(b["fu"], p["fu"]))
+            print(f"level {level} lds {lds} R {R}: velocity vs FP64 {eu:.3e}, vs psi kernel {ep:.3e}")
+            assert eu < 1e-6 and ep < 1e-6, (lds, R)
+    c.close()
+
+
 def test_graph_replay_matches_eager(omesh):
     """Steps replayed from the captured hipGraph give the same bits as eager
     launches, replay after replay with eager work in between."""
@@ -174,7 +202,7 @@ def test_graph_replay_matches_eager(omesh):
     b.close()
 
 
-@pytest.mark.parametrize("R,splits", [(1, 1), (2, 3), (4, 0), (4, 7)])
+@pytest.mark.parametrize("R,splits", [(1, 1), (2, 3), (3, 0), (3, 5), (4, 0), (4, 7)])
 def test_launch_shapes_agree(omesh, R, splits):
     m = omesh(4)
     lm = LeafMesh(m)
@@ -358,7 +386,7 @@ def test_error_paths():
     with pytest.raises(L.LpmError):  # FP32C needs the renormalised scheme
         c.advance(0.01, TWO_PI, L.SCHEME_REF, L.PREC_FP32C)
     with pytest.raises(L.LpmError):
-        c.set_launch(3, 0)
+        c.set_launch(5, 0)
     with pytest.raises(L.LpmError):
         c.advance(float("nan"), TWO_PI)
     c.close()
