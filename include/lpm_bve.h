@@ -132,6 +132,34 @@ int lpm_bve_destroy(lpm_bve_ctx* ctx);
 int lpm_bve_set_state(lpm_bve_ctx* ctx, int64_t nv, const double* vx, const double* vzeta, int64_t nf,
                       const double* fx, const double* fzeta, const double* farea, const uint8_t* fmask);
 
+/* Vorticity field evaluated on the device: the composite of lpm_ic_eval
+ * (solid-body rotation scaled by sbr, Rossby-Haurwitz wave rh_n > 0,
+ * nvort Gaussian vortices with host arrays centers [nvort][3], amps [nvort]). */
+typedef struct lpm_ic_params {
+  double sbr;
+  int rh_n;
+  double rh_w, rh_K;
+  int nvort;
+  const double* centers;
+  const double* amps;
+  double sigma;
+  int subtract_mean; /* subtract the area-weighted mean over active faces */
+} lpm_ic_params;
+
+/* The state of MeshSeed<IcosTriSphereSeed> + treeInit(level) +
+ * init_vorticity(ic) (src/LpmPolyMesh2d.cpp:30-44, src/LpmBVESphere.cpp:126-149)
+ * without host arrays: the level's mesh comes from a per-device cache in
+ * device memory (built by lpm_icos_tree_init's code and uploaded on first use,
+ * shared read-only by every context on the device) and the field is
+ * evaluated on the device, in FP64 (values agree with lpm_ic_eval's to
+ * rounding, not bit for bit).  Positions, numbering and areas are those of
+ * lpm_icos_tree_init(level, NULL, ...); nf counts every tree face. */
+int lpm_bve_set_state_level(lpm_bve_ctx* ctx, int level, const lpm_ic_params* ic);
+
+/* Free the cached meshes of `device` (all devices if device < 0) that no
+ * context uses; *kept (may be NULL) receives the number still in use. */
+int lpm_mesh_cache_clear(int device, int* kept);
+
 /* Provide the entry velocity (the BVERK4 precondition, F1: velocity valid at
  * the current state).  vu [nv][3], fu [nf][3]. */
 int lpm_bve_set_velocity(lpm_bve_ctx* ctx, const double* vu, const double* fu);

@@ -25,6 +25,15 @@ _bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
+
+
+class IcParams(C.Structure):
+    """lpm_ic_params of include/lpm_bve.h."""
+    _fields_ = [("sbr", C.c_double), ("rh_n", C.c_int), ("rh_w", C.c_double), ("rh_K", C.c_double),
+                ("nvort", C.c_int), ("centers", C.c_void_p), ("amps", C.c_void_p), ("sigma", C.c_double),
+                ("subtract_mean", C.c_int)]
+
+
 # (name, argtypes) for every entry point of include/lpm_bve.h
 SIGNATURES = {
     "lpm_last_error": [],
@@ -46,6 +55,8 @@ SIGNATURES = {
     "lpm_bve_create": [C.c_int, C.POINTER(C.c_void_p)],
     "lpm_bve_destroy": [C.c_void_p],
     "lpm_bve_set_state": [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _bp],
+    "lpm_bve_set_state_level": [C.c_void_p, C.c_int, C.POINTER(IcParams)],
+    "lpm_mesh_cache_clear": [C.c_int, C.POINTER(C.c_int)],
     "lpm_bve_set_velocity": [C.c_void_p, _dp, _dp],
     "lpm_bve_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int],
     "lpm_bve_advance": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],

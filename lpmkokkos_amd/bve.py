@@ -192,6 +192,19 @@ class Context:
              np.ascontiguousarray(fzeta, np.float64), np.ascontiguousarray(farea, np.float64),
              np.ascontiguousarray(fmask, np.uint8))
 
+    def set_state_level(self, level: int, ic: "VorticityInitialCondition", subtract_mean: bool = False):
+        """State of the level-`level` icosahedral tree (every tree face) from the
+        device-resident mesh cache, with `ic` evaluated on the device."""
+        nvort = 0 if ic.amps is None else len(ic.amps)
+        centers = np.ascontiguousarray(ic.centers, np.float64) if nvort else None
+        amps = np.ascontiguousarray(ic.amps, np.float64) if nvort else None
+        p = _lib.IcParams(float(ic.sbr), int(ic.rh_n), float(ic.rh_w), float(ic.rh_K), nvort,
+                          centers.ctypes.data if nvort else None, amps.ctypes.data if nvort else None,
+                          float(ic.sigma), int(subtract_mean))
+        call("lpm_bve_set_state_level", self.h, int(level), C.byref(p))
+        sizes = self.sizes()
+        self.nv, self.nf = sizes[0], sizes[1]
+
     def set_velocity(self, vu, fu):
         call("lpm_bve_set_velocity", self.h, np.ascontiguousarray(vu, np.float64),
              np.ascontiguousarray(fu, np.float64))

@@ -14,6 +14,7 @@
 
 #include "lpm_bve.h"
 #include "lpm_bve_kernels.hpp"
+#include "lpm_mesh.hpp"
 #include "lpm_status.hpp"
 
 namespace lpm {
@@ -1001,7 +1002,110 @@ __global__ __launch_bounds__(1024) void k_compact(const uint8_t* __restrict__ ma
   }
 }
 
+// ---------------------------------------------------------------------------
+// initial conditions on the device (the composite field of lpm_ic_eval:
+// solid-body rotation, Rossby-Haurwitz wave, Gaussian vortices, summed in
+// that order; BVESphere::init_vorticity, src/LpmBVESphere.cpp:126-149)
+// ---------------------------------------------------------------------------
+struct IcArgs {
+  double sbr;
+  int rh_n;
+  double rh_w, rh_K;
+  int nvort;
+  const double4* vort;  // {centre xyz, amplitude} per vortex
+  double s2;            // sigma^2
+};
+
+// atan4 (src/LpmUtilities.hpp:14-46): longitude in [0, 2 pi)
+__device__ double atan4_dev(double y, double x) {
+  if (x == 0.0) return y > 0.0 ? 0.5 * kPi : (y < 0.0 ? 1.5 * kPi : 0.0);
+  if (y == 0.0) return x > 0.0 ? 0.0 : kPi;
+  const double th = atan2(fabs(y), fabs(x));
+  if (x > 0.0 && y > 0.0) return th;
+  if (x < 0.0 && y > 0.0) return kPi - th;
+  if (x < 0.0 && y < 0.0) return kPi + th;
+  return 2.0 * kPi - th;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ic_eval(Pos3 pos, int n, IcArgs a, double* __restrict__ zeta) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double x = pos.at(i, 0), y = pos.at(i, 1), z = pos.at(i, 2);
+  double zt = 0.0;
+  if (a.sbr != 0.0) zt += a.sbr * (2 * (2 * kPi) * z);
+  if (a.rh_n > 0) {
+    const double lat = atan2(z, sqrt(x * x + y * y)), lon = atan4_dev(y, x);
+    const double cl = cos(lat), sl = sin(lat);
+    zt += 2 * a.rh_w * sl - a.rh_K * (a.rh_n + 1) * (a.rh_n + 2) * pow(cl, (double)a.rh_n) * sl * cos(a.rh_n * lon);
+  }
+  for (int k = 0; k < a.nvort; ++k) {
+    const double4 v = a.vort[k];
+    zt += v.w * exp(-(1.0 - (x * v.x + y * v.y + z * v.z)) / a.s2);
+  }
+  zeta[i] = zt;
+}
+
+// Area-weighted moments of zeta in a fixed order: workgroup b sums the
+// contiguous chunk b of the targets (lane-strided, then a fixed tree).
+constexpr int kMomentBlocks = 256;
+__global__ __launch_bounds__(kBlock) void k_zeta_moments(const double* __restrict__ zeta,
+                                                         const double* __restrict__ area, int n,
+                                                         double2* __restrict__ part) {
+  __shared__ double r0[kBlock / 64], r1[kBlock / 64];
+  const int chunk = (n + kMomentBlocks - 1) / kMomentBlocks;
+  const int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double za = 0.0, aa = 0.0;
+  for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
+    za += zeta[i] * area[i];
+    aa += area[i];
+  }
+  za = wave_sum(za);
+  aa = wave_sum(aa);
+  if ((threadIdx.x & 63) == 0) {
+    r0[threadIdx.x >> 6] = za;
+    r1[threadIdx.x >> 6] = aa;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = make_double2((r0[0] + r0[1]) + (r0[2] + r0[3]), (r1[0] + r1[1]) + (r1[2] + r1[3]));
+}
+
+// mean = sum(zeta A) / sum(A) over the partials in order
+__global__ void k_zeta_mean(const double2* __restrict__ part, double* __restrict__ mean) {
+  if (threadIdx.x != 0) return;
+  double za = 0.0, aa = 0.0;
+  for (int b = 0; b < kMomentBlocks; ++b) {
+    za += part[b].x;
+    aa += part[b].y;
+  }
+  *mean = za / aa;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sub_mean(double* __restrict__ zeta, int n,
+                                                     const double* __restrict__ mean) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) zeta[i] -= *mean;
+}
+
 }  // namespace dev
+
+// ---------------------------------------------------------------------------
+// device-resident leaf meshes, one per (device, level), shared by contexts
+// ---------------------------------------------------------------------------
+struct MeshEntry {
+  int device = 0, level = 0;
+  int64_t nv = 0, nf = 0;
+  int ns = 0, nt = 0;
+  std::vector<int64_t> fmap;        // active face -> tree face
+  std::vector<double> face_x_host;  // every tree face [nf][3]
+  std::vector<uint8_t> fmask;       // every tree face: 1 = divided
+  double* pos = nullptr;            // [3][nt]: vertices, then active faces in tree order
+  double* area = nullptr;           // [nt], 0 for vertices
+  int* smap = nullptr;              // source slot -> target index
+  int* self_slot = nullptr;         // target -> source slot or -1
+  int users = 0;
+  size_t bytes = 0;
+};
 
 // ===========================================================================
 // host side
@@ -1380,16 +1484,179 @@ struct lpm_bve_ctx {
   // pair-kernel timing
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timing_events;
   bool timing = false;
+  // state built from a cached device mesh (lpm_bve_set_state_level): the
+  // mesh's read-only arrays are shared, and the divided faces' vorticity is
+  // evaluated on the host from the field parameters when first read back
+  lpm::MeshEntry* mesh = nullptr;
+  bool face_zeta_pending = false;
+  double ic_sbr = 0.0, ic_rh_w = 0.0, ic_rh_K = 0.0, ic_sigma = 1.0;
+  int ic_rh_n = 0;
+  bool ic_centered = false;
+  double* ic_mean = nullptr;  // device scalar: subtracted mean
+  std::vector<double> ic_centers, ic_amps;
 };
 
 namespace lpm {
 namespace {
+
+std::mutex g_mesh_mu;
+std::map<std::pair<int, int>, MeshEntry*> g_meshes;
+
+void release_mesh(lpm_bve_ctx* c) {
+  c->face_zeta_pending = false;
+  c->ic_mean = nullptr;
+  if (!c->mesh) return;
+  std::lock_guard<std::mutex> g(g_mesh_mu);
+  --c->mesh->users;
+  c->mesh = nullptr;
+}
+
+void free_ctx_memory(lpm_bve_ctx* c);
+
+// Device arrays of a context's state for c->nt targets and c->ns sources.
+// own_static: also the read-only area, source map and self slots (else they
+// come from a cached mesh).
+int ctx_alloc_state(lpm_bve_ctx* c, bool own_static) {
+  const int nt = c->nt;
+  auto alloc = [&](void** p, size_t n) -> int {
+    if (hipMalloc(p, std::max<size_t>(n, 256)) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
+    c->allocations.push_back(*p);
+    return LPM_OK;
+  };
+  const size_t d = sizeof(double) * (size_t)nt;
+  int rc = 0;
+  // position, stage-argument and velocity triples are contiguous [3][nt]
+  // blocks so kernels address them as one SoA array
+  double** triples[] = {&c->x, &c->wx, &c->u};
+  for (double** p : triples)
+    if ((rc = alloc((void**)p, 3 * d))) return rc;
+  c->y = c->x + nt;
+  c->z = c->x + 2 * (size_t)nt;
+  c->wy = c->wx + nt;
+  c->wz = c->wx + 2 * (size_t)nt;
+  c->v = c->u + nt;
+  c->w = c->u + 2 * (size_t)nt;
+  double** singles[] = {&c->zeta, &c->psi, &c->absv, &c->wzeta};
+  for (double** p : singles)
+    if ((rc = alloc((void**)p, d))) return rc;
+  for (int s = 0; s < 4; ++s) {
+    if ((rc = alloc((void**)&c->kx[s], 3 * d))) return rc;
+    if ((rc = alloc((void**)&c->kz[s], d))) return rc;
+  }
+  if (own_static) {
+    if ((rc = alloc((void**)&c->area, d))) return rc;
+    if ((rc = alloc((void**)&c->smap, sizeof(int) * c->ns))) return rc;
+    if ((rc = alloc((void**)&c->self_slot, sizeof(int) * nt))) return rc;
+  }
+  if ((rc = alloc((void**)&c->circ, 256))) return rc;
+  return LPM_OK;
+}
+
+// The cached level-`level` mesh on `device`, built and uploaded on first use;
+// counts one more user.
+int get_mesh(int device, int level, MeshEntry** out) {
+  std::lock_guard<std::mutex> g(g_mesh_mu);
+  auto key = std::make_pair(device, level);
+  auto it = g_meshes.find(key);
+  if (it != g_meshes.end()) {
+    ++it->second->users;
+    *out = it->second;
+    return LPM_OK;
+  }
+  MeshEntry* m = new MeshEntry();
+  m->device = device;
+  m->level = level;
+  try {
+    const TreeMesh t = build_icos_tree(level, builtin_icos_seed());
+    m->nv = t.nverts();
+    m->nf = t.nfaces();
+    m->fmask = t.face_mask;
+    for (int64_t f = 0; f < m->nf; ++f)
+      if (!t.face_mask[f]) m->fmap.push_back(f);
+    m->ns = (int)m->fmap.size();
+    m->nt = (int)(m->nv + m->ns);
+    m->face_x_host.resize(3 * (size_t)m->nf);
+    std::memcpy(m->face_x_host.data(), t.face_phys.data(), sizeof(double) * 3 * m->nf);
+    const int nt = m->nt;
+    std::vector<double> hp(3 * (size_t)nt), ha(nt, 0.0);
+    std::vector<int> hs(m->ns), hself(nt, -1);
+    for (int64_t i = 0; i < m->nv; ++i)
+      for (int k = 0; k < 3; ++k) hp[k * (size_t)nt + i] = t.vert_phys[i][k];
+    for (int j = 0; j < m->ns; ++j) {
+      const int64_t f = m->fmap[j], i = m->nv + j;
+      for (int k = 0; k < 3; ++k) hp[k * (size_t)nt + i] = t.face_phys[f][k];
+      ha[i] = t.face_area[f];
+      hs[j] = (int)i;
+      hself[i] = j;
+    }
+    const size_t d = sizeof(double) * (size_t)nt;
+    m->bytes = 4 * d + sizeof(int) * ((size_t)m->ns + nt);
+    bool okm = hipMalloc(&m->pos, 3 * d) == hipSuccess;
+    okm = okm && hipMalloc(&m->area, d) == hipSuccess;
+    okm = okm && hipMalloc(&m->smap, sizeof(int) * m->ns) == hipSuccess;
+    okm = okm && hipMalloc(&m->self_slot, sizeof(int) * nt) == hipSuccess;
+    if (!okm) {
+      (void)hipFree(m->pos);
+      (void)hipFree(m->area);
+      (void)hipFree(m->smap);
+      (void)hipFree(m->self_slot);
+      delete m;
+      return fail(LPM_ERR_NOMEM, "mesh cache: device allocation failed");
+    }
+    LPM_HIP(hipMemcpy(m->pos, hp.data(), 3 * d, hipMemcpyHostToDevice));
+    LPM_HIP(hipMemcpy(m->area, ha.data(), d, hipMemcpyHostToDevice));
+    LPM_HIP(hipMemcpy(m->smap, hs.data(), sizeof(int) * m->ns, hipMemcpyHostToDevice));
+    LPM_HIP(hipMemcpy(m->self_slot, hself.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+  } catch (const std::exception& e) {
+    delete m;
+    return fail(LPM_ERR_NOMEM, std::string("mesh cache: ") + e.what());
+  }
+  m->users = 1;
+  g_meshes[key] = m;
+  *out = m;
+  return LPM_OK;
+}
+
+const std::vector<int64_t>& ctx_fmap(const lpm_bve_ctx* c) { return c->mesh ? c->mesh->fmap : c->fmap; }
+
+// Host copy of every tree face's vorticity: the divided faces' values (the
+// field at their positions, less the subtracted mean) for a cached-mesh state.
+int ctx_face_zeta_host(lpm_bve_ctx* c) {
+  if (!c->face_zeta_pending) return LPM_OK;
+  double mean = 0.0;
+  if (c->ic_centered) {
+    LPM_HIP(hipMemcpyAsync(&mean, c->ic_mean, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    LPM_HIP(hipStreamSynchronize(c->stream));
+  }
+  const MeshEntry* m = c->mesh;
+  std::vector<int64_t> pf;
+  std::vector<double> px;
+  for (int64_t f = 0; f < c->nf; ++f)
+    if (m->fmask[f]) {
+      pf.push_back(f);
+      for (int k = 0; k < 3; ++k) px.push_back(m->face_x_host[3 * f + k]);
+    }
+  std::vector<double> pz(pf.size());
+  const int nvort = (int)c->ic_amps.size();
+  if (!pf.empty()) {
+    const int rc = lpm_ic_eval(px.data(), (int64_t)pf.size(), pz.data(), c->ic_sbr, c->ic_rh_n, c->ic_rh_w,
+                               c->ic_rh_K, nvort, nvort ? c->ic_centers.data() : nullptr,
+                               nvort ? c->ic_amps.data() : nullptr, c->ic_sigma);
+    if (rc) return rc;
+  }
+  // active faces are filled from the device state by the caller
+  c->face_zeta_host.assign(c->nf, 0.0);
+  for (size_t p = 0; p < pf.size(); ++p) c->face_zeta_host[pf[p]] = pz[p] - mean;
+  c->face_zeta_pending = false;
+  return LPM_OK;
+}
 
 void free_ctx_memory(lpm_bve_ctx* c) {
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   c->graphs.clear();
   for (void* p : c->allocations) (void)hipFree(p);
   c->allocations.clear();
+  release_mesh(c);
   for (auto& e : c->timing_events) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
@@ -1639,34 +1906,9 @@ int lpm_bve_set_state(lpm_bve_ctx* c, int64_t nv, const double* vx, const double
     hsmap[j] = (int)i;
     hself[i] = j;
   }
-  auto alloc = [&](void** p, size_t n) -> int {
-    if (hipMalloc(p, std::max<size_t>(n, 256)) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
-    c->allocations.push_back(*p);
-    return LPM_OK;
-  };
+  int rc = ctx_alloc_state(c, true);
+  if (rc) return rc;
   const size_t d = sizeof(double) * (size_t)nt;
-  int rc = 0;
-  // position, stage-argument and velocity triples are contiguous [3][nt]
-  // blocks so kernels address them as one SoA array
-  double** triples[] = {&c->x, &c->wx, &c->u};
-  for (double** p : triples)
-    if ((rc = alloc((void**)p, 3 * d))) return rc;
-  c->y = c->x + nt;
-  c->z = c->x + 2 * (size_t)nt;
-  c->wy = c->wx + nt;
-  c->wz = c->wx + 2 * (size_t)nt;
-  c->v = c->u + nt;
-  c->w = c->u + 2 * (size_t)nt;
-  double** singles[] = {&c->zeta, &c->psi, &c->area, &c->absv, &c->wzeta};
-  for (double** p : singles)
-    if ((rc = alloc((void**)p, d))) return rc;
-  for (int s = 0; s < 4; ++s) {
-    if ((rc = alloc((void**)&c->kx[s], 3 * d))) return rc;
-    if ((rc = alloc((void**)&c->kz[s], d))) return rc;
-  }
-  if ((rc = alloc((void**)&c->smap, sizeof(int) * c->ns))) return rc;
-  if ((rc = alloc((void**)&c->self_slot, sizeof(int) * nt))) return rc;
-  if ((rc = alloc((void**)&c->circ, 256))) return rc;
   LPM_HIP(hipMemcpyAsync(c->x, h.data(), d, hipMemcpyHostToDevice, c->stream));
   LPM_HIP(hipMemcpyAsync(c->y, h.data() + nt, d, hipMemcpyHostToDevice, c->stream));
   LPM_HIP(hipMemcpyAsync(c->z, h.data() + 2 * (size_t)nt, d, hipMemcpyHostToDevice, c->stream));
@@ -1681,6 +1923,102 @@ int lpm_bve_set_state(lpm_bve_ctx* c, int64_t nv, const double* vx, const double
   return ok();
 }
 
+int lpm_bve_set_state_level(lpm_bve_ctx* c, int level, const lpm_ic_params* ic) {
+  if (!c || !ic) return fail(LPM_ERR_ARG, "lpm_bve_set_state_level: null argument");
+  if (level < 0 || level > 10) return fail(LPM_ERR_ARG, "lpm_bve_set_state_level: level must be in [0, 10]");
+  if (ic->nvort < 0 || (ic->nvort > 0 && (!ic->centers || !ic->amps)) || !(ic->sigma > 0.0))
+    return fail(LPM_ERR_ARG, "lpm_bve_set_state_level: bad field parameters");
+  LPM_HIP(hipSetDevice(c->device));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  free_ctx_memory(c);
+  c->eb = EvalBuffers();
+  MeshEntry* m = nullptr;
+  int rc = get_mesh(c->device, level, &m);
+  if (rc) return rc;
+  c->mesh = m;
+  c->nv = m->nv;
+  c->nf = m->nf;
+  c->ns = m->ns;
+  c->nt = m->nt;
+  c->fmap.clear();
+  c->face_x_host.clear();
+  c->face_zeta_host.clear();
+  if ((rc = ctx_alloc_state(c, false))) return rc;
+  c->area = m->area;
+  c->smap = m->smap;
+  c->self_slot = m->self_slot;
+  const int nt = c->nt;
+  LPM_HIP(hipMemcpyAsync(c->x, m->pos, 3 * sizeof(double) * (size_t)nt, hipMemcpyDeviceToDevice, c->stream));
+  // field parameters: vortices as {centre, amplitude}; the subtracted mean
+  const int nv = ic->nvort;
+  std::vector<double4> hv(std::max(nv, 1));
+  for (int k = 0; k < nv; ++k)
+    hv[k] = make_double4(ic->centers[3 * k], ic->centers[3 * k + 1], ic->centers[3 * k + 2], ic->amps[k]);
+  double4* dv = nullptr;
+  double* scratch = nullptr;
+  if (hipMalloc(&dv, sizeof(double4) * hv.size()) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
+  c->allocations.push_back(dv);
+  const size_t scratch_bytes = sizeof(double2) * dev::kMomentBlocks + 256;
+  if (hipMalloc(&scratch, scratch_bytes) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
+  c->allocations.push_back(scratch);
+  LPM_HIP(hipMemcpyAsync(dv, hv.data(), sizeof(double4) * hv.size(), hipMemcpyHostToDevice, c->stream));
+  IcArgs a;
+  a.sbr = ic->sbr;
+  a.rh_n = ic->rh_n;
+  a.rh_w = ic->rh_w;
+  a.rh_K = ic->rh_K;
+  a.nvort = nv;
+  a.vort = dv;
+  a.s2 = ic->sigma * ic->sigma;
+  const dim3 grid(cdiv(nt, kBlock));
+  hipLaunchKernelGGL(k_ic_eval, grid, dim3(kBlock), 0, c->stream, Pos3{c->x, 1, (long)nt}, nt, a, c->zeta);
+  double* mean = scratch;
+  double2* part = reinterpret_cast<double2*>(scratch + 32);
+  c->ic_centered = ic->subtract_mean != 0;
+  if (c->ic_centered) {
+    hipLaunchKernelGGL(k_zeta_moments, dim3(kMomentBlocks), dim3(kBlock), 0, c->stream, c->zeta, c->area, nt, part);
+    hipLaunchKernelGGL(k_zeta_mean, dim3(1), dim3(64), 0, c->stream, part, mean);
+    hipLaunchKernelGGL(k_sub_mean, grid, dim3(kBlock), 0, c->stream, c->zeta, nt, mean);
+  }
+  LPM_HIP(hipGetLastError());
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  c->ic_mean = mean;
+  c->ic_sbr = ic->sbr;
+  c->ic_rh_n = ic->rh_n;
+  c->ic_rh_w = ic->rh_w;
+  c->ic_rh_K = ic->rh_K;
+  c->ic_sigma = ic->sigma;
+  c->ic_centers.assign(ic->centers, ic->centers + 3 * (size_t)nv);
+  c->ic_amps.assign(ic->amps, ic->amps + nv);
+  c->face_zeta_pending = true;
+  c->vel_valid = false;
+  c->psi_valid = false;
+  c->absv_valid = false;
+  return ok();
+}
+
+int lpm_mesh_cache_clear(int device, int* kept) {
+  std::lock_guard<std::mutex> g(g_mesh_mu);
+  int n_kept = 0;
+  for (auto it = g_meshes.begin(); it != g_meshes.end();) {
+    MeshEntry* m = it->second;
+    if ((device >= 0 && m->device != device) || m->users > 0) {
+      n_kept += m->users > 0;
+      ++it;
+      continue;
+    }
+    (void)hipSetDevice(m->device);
+    (void)hipFree(m->pos);
+    (void)hipFree(m->area);
+    (void)hipFree(m->smap);
+    (void)hipFree(m->self_slot);
+    delete m;
+    it = g_meshes.erase(it);
+  }
+  if (kept) *kept = n_kept;
+  return ok();
+}
+
 int lpm_bve_set_velocity(lpm_bve_ctx* c, const double* vu, const double* fu) {
   if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_set_velocity: no state");
   if ((c->nv && !vu) || !fu) return fail(LPM_ERR_ARG, "lpm_bve_set_velocity: null array");
@@ -1689,8 +2027,9 @@ int lpm_bve_set_velocity(lpm_bve_ctx* c, const double* vu, const double* fu) {
   std::vector<double> h(3 * (size_t)nt);
   for (int64_t i = 0; i < c->nv; ++i)
     for (int k = 0; k < 3; ++k) h[k * (size_t)nt + i] = vu[3 * i + k];
+  const std::vector<int64_t>& fmap = ctx_fmap(c);
   for (int j = 0; j < c->ns; ++j)
-    for (int k = 0; k < 3; ++k) h[k * (size_t)nt + c->nv + j] = fu[3 * c->fmap[j] + k];
+    for (int k = 0; k < 3; ++k) h[k * (size_t)nt + c->nv + j] = fu[3 * fmap[j] + k];
   const size_t d = sizeof(double) * nt;
   LPM_HIP(hipMemcpyAsync(c->u, h.data(), d, hipMemcpyHostToDevice, c->stream));
   LPM_HIP(hipMemcpyAsync(c->v, h.data() + nt, d, hipMemcpyHostToDevice, c->stream));
@@ -1777,12 +2116,18 @@ int lpm_bve_get_state(lpm_bve_ctx* c, double* vx, double* vzeta, double* vu, dou
     if (vpsi) vpsi[i] = col(7, i);
   }
   // divided faces: stored position and vorticity, zero velocity and psi
-  if (fx) std::memcpy(fx, c->face_x_host.data(), sizeof(double) * 3 * c->nf);
+  if (fzeta) {
+    const int rc = ctx_face_zeta_host(c);
+    if (rc) return rc;
+  }
+  const std::vector<double>& fxh = c->mesh ? c->mesh->face_x_host : c->face_x_host;
+  if (fx) std::memcpy(fx, fxh.data(), sizeof(double) * 3 * c->nf);
   if (fzeta) std::memcpy(fzeta, c->face_zeta_host.data(), sizeof(double) * c->nf);
   if (fu) std::memset(fu, 0, sizeof(double) * 3 * c->nf);
   if (fpsi) std::memset(fpsi, 0, sizeof(double) * c->nf);
+  const std::vector<int64_t>& fmap = ctx_fmap(c);
   for (int j = 0; j < c->ns; ++j) {
-    const int64_t f = c->fmap[j], i = c->nv + j;
+    const int64_t f = fmap[j], i = c->nv + j;
     for (int k = 0; k < 3; ++k) {
       if (fx) fx[3 * f + k] = col(k, i);
       if (fu) fu[3 * f + k] = col(4 + k, i);

@@ -116,20 +116,32 @@ def leaf_mesh(level: int):
     return m, np.ascontiguousarray(m.fx[lv]), np.ascontiguousarray(m.farea[lv])
 
 
-def member_state(spec: MemberSpec):
-    m, fx, farea = leaf_mesh(spec.level)
+def member_ic(spec: MemberSpec) -> VorticityInitialCondition:
+    """RH wave (w = 0.1, K = 0.1 k_factor) plus the member's seeded vortices."""
     ic = VorticityInitialCondition(rh_n=spec.rh_n, rh_w=0.1, rh_K=0.1 * spec.k_factor)
     if spec.nvort:
         ic.with_vortices(spec.vortex_seed, spec.nvort, spec.amp, spec.sigma)
+    return ic
+
+
+def member_state(spec: MemberSpec):
+    """Host-built state (leaf faces only), mean removed."""
+    m, fx, farea = leaf_mesh(spec.level)
+    ic = member_ic(spec)
     vz = ic.eval_many(m.vx)
     fz = ic.eval_many(fx)
     mu = area_mean(fz, farea, np.zeros(len(fz), np.uint8))
     return m.vx, vz - mu, fx, fz - mu, farea
 
 
-def make_member(spec: MemberSpec, device: int = 0) -> Context:
-    vx, vz, fx, fz, farea = member_state(spec)
+def make_member(spec: MemberSpec, device: int = 0, on_device: bool = True) -> Context:
+    """A context holding the member's initial state, mean removed: built on the
+    device from the cached level mesh (default), or on the host and uploaded."""
     c = Context(device)
+    if on_device:
+        c.set_state_level(spec.level, member_ic(spec), subtract_mean=True)
+        return c
+    vx, vz, fx, fz, farea = member_state(spec)
     c.set_state(vx, vz, fx, fz, farea, np.zeros(len(fz), np.uint8))
     return c
 

@@ -6,9 +6,10 @@ arrived while a slot and the device-memory budget allow, keeps each admitted
 request's particle state resident in HBM (one context per request), advances
 all of them together -- one ensemble step is four persistent pair kernels over
 the flattened work of every active request -- and retires requests that have
-completed their steps, freeing their memory for the next arrivals.  Host
-preparation of new arrivals (mesh from the per-level cache, initial field,
-upload) overlaps the device step in flight.
+completed their steps, freeing their memory for the next arrivals.  A new
+request's state is built on the device: positions copied from the per-level
+mesh cached in device memory, the initial field evaluated by a kernel, so
+admission needs no host mesh, no host field evaluation and no upload.
 
 Each request's result is bit-identical to running it alone: a member keeps
 the decomposition it has alone and writes only its own partial slots.
@@ -31,8 +32,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import PREC_FP32C, SCHEME_LAGR
-from .bve import Context
-from .ensemble import Ensemble, MemberSpec, leaf_mesh, make_member, rng_draws, state_bytes
+from .bve import Context, VorticityInitialCondition
+from .ensemble import Ensemble, MemberSpec, make_member, rng_draws, state_bytes
 
 LEVEL_PROBS = {4: 0.30, 5: 0.30, 6: 0.20, 7: 0.15, 8: 0.05}
 
@@ -49,12 +50,17 @@ class Request:
     admitted: float = math.nan
     finished: float = math.nan
     steps_done: int = 0
+    admit_s: float = 0.0  # wall time of building the request's state
     pairs: float = 0.0
     result: dict | None = field(default=None, repr=False)
 
     @property
     def latency(self) -> float:
         return self.finished - self.arrival
+
+
+def make_ic_zero() -> VorticityInitialCondition:
+    return VorticityInitialCondition()
 
 
 def pairs_per_step(level: int) -> float:
@@ -102,8 +108,17 @@ class Server:
         self.used = 0
         self.log = log or (lambda s: None)
 
+    def warm(self, levels):
+        """Build the device mesh cache for `levels` (once per process)."""
+        for lev in sorted(set(levels)):
+            c = Context(self.device)
+            c.set_state_level(lev, make_ic_zero())
+            c.close()
+
     def _admit(self, r: Request, now: float):
+        t = time.perf_counter()
         r.ctx = make_member(r.spec, self.device)
+        r.admit_s = time.perf_counter() - t
         r.bytes = request_bytes(r.spec)
         self.used += r.bytes
         r.admitted = now
@@ -126,6 +141,7 @@ class Server:
         for r in requests:
             if request_bytes(r.spec) > self.budget:
                 raise ValueError(f"request {r.rid} (level {r.spec.level}) exceeds the memory budget")
+        self.warm(r.spec.level for r in requests)
         pending = sorted(requests, key=lambda r: r.arrival)
         done: list[Request] = []
         t0 = time.perf_counter()
@@ -145,11 +161,6 @@ class Server:
                 continue
             ts = time.perf_counter()
             self.ens.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=1)
-            # prepare the next arrivals' meshes while the step runs
-            lookahead = time.perf_counter() - t0 + 0.5
-            for r in pending[:4]:
-                if r.arrival <= lookahead:
-                    leaf_mesh(r.spec.level)
             self.ens.synchronize()
             te = time.perf_counter()
             busy += te - ts
@@ -176,6 +187,8 @@ class Server:
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()),
+            "admission_mean_s": float(np.mean([r.admit_s for r in done])),
+            "admission_max_s": float(np.max([r.admit_s for r in done])),
             "levels": {int(l): int(sum(1 for r in done if r.spec.level == l)) for l in sorted({r.spec.level for r in done})},
         }
 
