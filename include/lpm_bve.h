@@ -124,6 +124,12 @@ int lpm_bve_face_streamfn(double* psi, const double* fx, const double* fzeta, co
 typedef struct lpm_bve_ctx lpm_bve_ctx;
 
 int lpm_bve_create(int device, lpm_bve_ctx** ctx);
+/* A context that queues its work on a caller's stream (a hipStream_t of
+ * `device`, not owned: it must outlive the context).  Contexts sharing a
+ * stream, e.g. the members of one ensemble on the ensemble's stream
+ * (lpm_ens_stream), are ordered by the stream and need no cross-stream
+ * events. */
+int lpm_bve_create_on_stream(int device, void* stream, lpm_bve_ctx** ctx);
 int lpm_bve_destroy(lpm_bve_ctx* ctx);
 
 /* Upload a particle state (host arrays; copied).  vx [nv][3], vzeta [nv];
@@ -185,6 +191,9 @@ int lpm_bve_circulation(lpm_bve_ctx* ctx, double* circ);
 
 /* Block until all work queued on the context has finished. */
 int lpm_bve_synchronize(lpm_bve_ctx* ctx);
+/* Non-blocking: *done = 1 when the context's asynchronous work (solves, steps,
+ * ensemble steps) queued so far has finished. */
+int lpm_bve_query(lpm_bve_ctx* ctx, int* done);
 
 /* Sizes of the resident problem: {nv, nf, nsources, ntargets}. */
 int lpm_bve_sizes(lpm_bve_ctx* ctx, int64_t* sizes4);
@@ -256,6 +265,10 @@ int lpm_ens_set_blocks_per_cu(lpm_bve_ensemble* ens, int blocks_per_cu);
 int lpm_ens_advance(lpm_bve_ensemble* ens, double dt, double omega, int scheme, int prec, int nsteps,
                     int use_graph);
 int lpm_ens_synchronize(lpm_bve_ensemble* ens);
+/* Non-blocking: *done = 1 when the steps queued by the ensemble have finished. */
+int lpm_ens_query(lpm_bve_ensemble* ens, int* done);
+/* The ensemble's stream (hipStream_t), for members created on it. */
+int lpm_ens_stream(lpm_bve_ensemble* ens, void** stream);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,7 @@ SIGNATURES = {
                                 C.c_int64, C.c_void_p],
     "lpm_bve_face_streamfn": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
     "lpm_bve_create": [C.c_int, C.POINTER(C.c_void_p)],
+    "lpm_bve_create_on_stream": [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)],
     "lpm_bve_destroy": [C.c_void_p],
     "lpm_bve_set_state": [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _bp],
     "lpm_bve_set_state_level": [C.c_void_p, C.c_int, C.POINTER(IcParams)],
@@ -63,6 +64,7 @@ SIGNATURES = {
     "lpm_bve_get_state": [C.c_void_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
     "lpm_bve_circulation": [C.c_void_p, C.POINTER(C.c_double)],
     "lpm_bve_synchronize": [C.c_void_p],
+    "lpm_bve_query": [C.c_void_p, C.POINTER(C.c_int)],
     "lpm_bve_sizes": [C.c_void_p, _i64],
     "lpm_bve_set_launch": [C.c_void_p, C.c_int, C.c_int],
     "lpm_bve_set_near_factor": [C.c_void_p, C.c_double],
@@ -84,6 +86,8 @@ SIGNATURES = {
     "lpm_ens_set_blocks_per_cu": [C.c_void_p, C.c_int],
     "lpm_ens_advance": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
     "lpm_ens_synchronize": [C.c_void_p],
+    "lpm_ens_query": [C.c_void_p, C.POINTER(C.c_int)],
+    "lpm_ens_stream": [C.c_void_p, C.POINTER(C.c_void_p)],
 }
 
 _lib = None

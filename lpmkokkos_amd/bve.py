@@ -166,9 +166,15 @@ def area_mean(fzeta, farea, fmask) -> float:
 class Context:
     """One lpm_bve_ctx: a particle state resident in device memory."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, stream=None):
+        """stream: a hipStream_t (int or c_void_p) to queue work on instead of
+        the context's own stream; it must outlive the context."""
         h = C.c_void_p()
-        call("lpm_bve_create", device, C.byref(h))
+        if stream is None:
+            call("lpm_bve_create", device, C.byref(h))
+        else:
+            call("lpm_bve_create_on_stream", device, C.c_void_p(int(stream.value if hasattr(stream, "value")
+                                                                    else stream)), C.byref(h))
         self.h = h
         self.device = device
         self.nv = self.nf = 0
@@ -235,6 +241,12 @@ class Context:
 
     def synchronize(self):
         call("lpm_bve_synchronize", self.h)
+
+    def done(self) -> bool:
+        """True when all work queued on the context has finished (non-blocking)."""
+        d = C.c_int()
+        call("lpm_bve_query", self.h, C.byref(d))
+        return bool(d.value)
 
     def sizes(self):
         s = np.zeros(4, np.int64)

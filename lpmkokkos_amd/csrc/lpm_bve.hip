@@ -789,6 +789,15 @@ __global__ void k_reset_counter(int* counter) {
   if (threadIdx.x == 0) *counter = 0;
 }
 
+// Completion marker: lane 0 writes `seq` to host-visible (pinned) memory after
+// everything queued before it on the stream.
+__global__ void k_mark(volatile unsigned long long* flag, unsigned long long seq) {
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    *flag = seq;
+  }
+}
+
 __device__ __forceinline__ int next_item(int* counter, int* slot) {
   if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
   __syncthreads();
@@ -1488,6 +1497,15 @@ struct lpm_bve_ctx {
   // mesh's read-only arrays are shared, and the divided faces' vorticity is
   // evaluated on the host from the field parameters when first read back
   lpm::MeshEntry* mesh = nullptr;
+  bool own_stream = true;
+  // completion marker behind the last asynchronous work queued for the
+  // context (lpm_bve_solve, lpm_bve_advance, ensemble steps): a one-lane
+  // kernel writes done_seq into pinned host memory; lpm_bve_query compares
+  unsigned long long* done_flag = nullptr;
+  unsigned long long done_seq = 0;
+  // the marker of the ensemble whose stream the context shares (its steps)
+  const unsigned long long* done_flag_shared = nullptr;
+  unsigned long long done_seq_shared = 0;
   bool face_zeta_pending = false;
   double ic_sbr = 0.0, ic_rh_w = 0.0, ic_rh_K = 0.0, ic_sigma = 1.0;
   int ic_rh_n = 0;
@@ -1512,6 +1530,25 @@ void release_mesh(lpm_bve_ctx* c) {
 }
 
 void free_ctx_memory(lpm_bve_ctx* c);
+
+// Queue a completion marker behind the work queued so far on `st`.
+int queue_mark(unsigned long long** flag, unsigned long long* seq, hipStream_t st) {
+  if (!*flag) {
+    if (hipHostMalloc((void**)flag, sizeof(unsigned long long), hipHostMallocCoherent) != hipSuccess)
+      return fail(LPM_ERR_NOMEM, "completion marker: pinned allocation failed");
+    **flag = 0;
+  }
+  ++*seq;
+  hipLaunchKernelGGL(k_mark, dim3(1), dim3(64), 0, st, *flag, *seq);
+  LPM_HIP(hipGetLastError());
+  return LPM_OK;
+}
+
+bool mark_reached(const unsigned long long* flag, unsigned long long seq) {
+  return !flag || *(const volatile unsigned long long*)flag >= seq;
+}
+
+int ctx_mark_done(lpm_bve_ctx* c) { return queue_mark(&c->done_flag, &c->done_seq, c->stream); }
 
 // Device arrays of a context's state for c->nt targets and c->ns sources.
 // own_static: also the read-only area, source map and self slots (else they
@@ -1856,12 +1893,26 @@ int lpm_bve_create(int device, lpm_bve_ctx** out) {
   return ok();
 }
 
+int lpm_bve_create_on_stream(int device, void* stream, lpm_bve_ctx** out) {
+  if (!out || !stream) return fail(LPM_ERR_ARG, "lpm_bve_create_on_stream: null argument");
+  int n = 0;
+  LPM_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(LPM_ERR_ARG, "lpm_bve_create_on_stream: no such device");
+  lpm_bve_ctx* c = new lpm_bve_ctx();
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  c->own_stream = false;
+  *out = c;
+  return ok();
+}
+
 int lpm_bve_destroy(lpm_bve_ctx* c) {
   if (!c) return ok();
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   free_ctx_memory(c);
-  (void)hipStreamDestroy(c->stream);
+  if (c->done_flag) (void)hipHostFree(c->done_flag);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ok();
 }
@@ -2053,6 +2104,7 @@ int lpm_bve_solve(lpm_bve_ctx* c, int prec, int scheme, int want_psi) {
   c->vel_prec = prec;
   c->vel_form = form;
   c->psi_valid = want_psi != 0;
+  if ((rc = ctx_mark_done(c))) return rc;
   return ok();
 }
 
@@ -2093,6 +2145,7 @@ int lpm_bve_advance(lpm_bve_ctx* c, double dt, double omega, int scheme, int pre
   c->vel_prec = prec;
   c->vel_form = form;
   c->psi_valid = false;
+  if ((rc = ctx_mark_done(c))) return rc;
   return ok();
 }
 
@@ -2243,12 +2296,20 @@ struct lpm_bve_ensemble {
   std::vector<int> plan_nitems;            // items per R launch
   std::vector<int> plan_grid;              // persistent workgroups per R launch
   std::vector<lpm::dev::EnsItem*> d_items;  // per R launch
+  // Plan buffers grow on demand and are kept (freed at destroy): a hipFree
+  // waits for the whole device, so a membership change while other work
+  // runs on the GPU must not free.
   lpm::dev::EnsMember* d_mem_w = nullptr;  // descriptors: stage arguments as targets
   lpm::dev::EnsMember* d_mem_x = nullptr;  // descriptors: state as targets
-  int* d_counters = nullptr;               // one per persistent launch in a step
-  std::vector<void*> plan_allocs;
+  size_t cap_mem = 0;
+  std::vector<lpm::dev::EnsItem*> item_bufs;  // by launch index
+  std::vector<size_t> item_caps;
+  int* d_counters = nullptr;  // four per persistent launch in a step
+  std::vector<void*> plan_allocs;  // every plan buffer ever allocated
   std::map<std::tuple<double, double, int, int>, hipGraphExec_t> graphs;
   hipEvent_t ev = nullptr;
+  unsigned long long* done_flag = nullptr;  // completion marker (lpm_ens_query)
+  unsigned long long done_seq = 0;
 };
 
 namespace lpm {
@@ -2257,15 +2318,21 @@ namespace {
 void ens_drop_plan(lpm_bve_ensemble* e) {
   for (auto& kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
   e->graphs.clear();
-  for (void* p : e->plan_allocs) (void)hipFree(p);
-  e->plan_allocs.clear();
   e->d_items.clear();
   e->plan_R.clear();
   e->plan_nitems.clear();
   e->plan_grid.clear();
-  e->d_mem_w = e->d_mem_x = nullptr;
-  e->d_counters = nullptr;
   e->plan_valid = false;
+}
+
+void ens_free_buffers(lpm_bve_ensemble* e) {
+  for (void* p : e->plan_allocs) (void)hipFree(p);
+  e->plan_allocs.clear();
+  e->d_mem_w = e->d_mem_x = nullptr;
+  e->cap_mem = 0;
+  e->item_bufs.clear();
+  e->item_caps.clear();
+  e->d_counters = nullptr;
 }
 
 bool ens_plan_current(const lpm_bve_ensemble* e, int prec, int form) {
@@ -2317,10 +2384,19 @@ int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
     return LPM_OK;
   };
   int rc;
-  if ((rc = alloc((void**)&e->d_mem_w, sizeof(EnsMember) * M))) return rc;
-  if ((rc = alloc((void**)&e->d_mem_x, sizeof(EnsMember) * M))) return rc;
-  LPM_HIP(hipMemcpy(e->d_mem_w, hw.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice));
-  LPM_HIP(hipMemcpy(e->d_mem_x, hx.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice));
+  if (M > e->cap_mem) {
+    // grow (doubling); the old buffers stay allocated until destroy
+    const size_t cap = std::max<size_t>(M, 2 * e->cap_mem);
+    if ((rc = alloc((void**)&e->d_mem_w, sizeof(EnsMember) * cap))) return rc;
+    if ((rc = alloc((void**)&e->d_mem_x, sizeof(EnsMember) * cap))) return rc;
+    e->cap_mem = cap;
+  }
+  if (!e->d_counters && (rc = alloc((void**)&e->d_counters, sizeof(int) * 4 * 4))) return rc;
+  // stream-ordered uploads: the previous step (which reads these buffers) is
+  // ahead of them on the ensemble's stream
+  LPM_HIP(hipMemcpyAsync(e->d_mem_w, hw.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice, e->stream));
+  LPM_HIP(hipMemcpyAsync(e->d_mem_x, hx.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice, e->stream));
+  size_t k = 0;
   for (auto& kv : byR) {
     auto& v = kv.second;
     std::stable_sort(v.begin(), v.end(), [](const std::pair<long, EnsItem>& a, const std::pair<long, EnsItem>& b) {
@@ -2329,15 +2405,25 @@ int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
     std::vector<EnsItem> items;
     items.reserve(v.size());
     for (auto& pr : v) items.push_back(pr.second);
-    EnsItem* d = nullptr;
-    if ((rc = alloc((void**)&d, sizeof(EnsItem) * items.size()))) return rc;
-    LPM_HIP(hipMemcpy(d, items.data(), sizeof(EnsItem) * items.size(), hipMemcpyHostToDevice));
+    if (k >= e->item_bufs.size()) {
+      e->item_bufs.push_back(nullptr);
+      e->item_caps.push_back(0);
+    }
+    if (items.size() > e->item_caps[k]) {
+      const size_t cap = std::max(items.size(), 2 * e->item_caps[k]);
+      if ((rc = alloc((void**)&e->item_bufs[k], sizeof(EnsItem) * cap))) return rc;
+      e->item_caps[k] = cap;
+    }
+    EnsItem* d = e->item_bufs[k];
+    LPM_HIP(hipMemcpyAsync(d, items.data(), sizeof(EnsItem) * items.size(), hipMemcpyHostToDevice, e->stream));
     e->plan_R.push_back(kv.first);
     e->plan_nitems.push_back((int)items.size());
     e->plan_grid.push_back(ens_grid(e, kv.first, prec, form, (int)items.size()));
     e->d_items.push_back(d);
+    ++k;
   }
-  if ((rc = alloc((void**)&e->d_counters, sizeof(int) * 4 * std::max<size_t>(1, e->plan_R.size())))) return rc;
+  // the host copies above must outlive the uploads
+  LPM_HIP(hipStreamSynchronize(e->stream));
   e->plan_generations.clear();
   for (auto* c : e->members) e->plan_generations.push_back(c->generation);
   e->plan_prec = prec;
@@ -2444,12 +2530,22 @@ int lpm_ens_create(int device, lpm_bve_ensemble** out) {
   return ok();
 }
 
+int lpm_ens_stream(lpm_bve_ensemble* e, void** stream) {
+  if (!e || !stream) return fail(LPM_ERR_ARG, "lpm_ens_stream: null argument");
+  *stream = (void*)e->stream;
+  return ok();
+}
+
 int lpm_ens_destroy(lpm_bve_ensemble* e) {
   if (!e) return ok();
   (void)hipSetDevice(e->device);
   (void)hipStreamSynchronize(e->stream);
+  for (auto* c : e->members)
+    if (c->done_flag_shared == e->done_flag) c->done_flag_shared = nullptr;
   ens_drop_plan(e);
+  ens_free_buffers(e);
   (void)hipEventDestroy(e->ev);
+  if (e->done_flag) (void)hipHostFree(e->done_flag);
   (void)hipStreamDestroy(e->stream);
   delete e;
   return ok();
@@ -2472,6 +2568,7 @@ int lpm_ens_remove(lpm_bve_ensemble* e, lpm_bve_ctx* c) {
   if (it == e->members.end()) return fail(LPM_ERR_ARG, "lpm_ens_remove: not a member");
   LPM_HIP(hipSetDevice(e->device));
   LPM_HIP(hipStreamSynchronize(e->stream));
+  if (c->done_flag_shared == e->done_flag) c->done_flag_shared = nullptr;  // its steps have finished
   e->members.erase(it);
   e->plan_valid = false;
   return ok();
@@ -2502,9 +2599,12 @@ int lpm_ens_advance(lpm_bve_ensemble* e, double dt, double omega, int scheme, in
   if (e->members.empty() || nsteps == 0) return ok();
   LPM_HIP(hipSetDevice(e->device));
   const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
-  // order after each member's earlier work, then prepare every member
+  // order after each member's earlier asynchronous work (its completion
+  // event, if still pending; members on the ensemble's stream are ordered by
+  // the stream), then prepare every member
   for (auto* c : e->members) {
     if (c->timing) return fail(LPM_ERR_STATE, "lpm_ens_advance: member has pair timing enabled");
+    if (c->stream == e->stream || mark_reached(c->done_flag, c->done_seq)) continue;
     LPM_HIP(hipEventRecord(e->ev, c->stream));
     LPM_HIP(hipStreamWaitEvent(e->stream, e->ev, 0));
   }
@@ -2536,8 +2636,16 @@ int lpm_ens_advance(lpm_bve_ensemble* e, double dt, double omega, int scheme, in
   }
   // later work on each member's own stream waits for the ensemble's
   LPM_HIP(hipEventRecord(e->ev, e->stream));
+  if ((rc = queue_mark(&e->done_flag, &e->done_seq, e->stream))) return rc;
   for (auto* c : e->members) {
-    LPM_HIP(hipStreamWaitEvent(c->stream, e->ev, 0));
+    if (c->stream != e->stream) {
+      LPM_HIP(hipStreamWaitEvent(c->stream, e->ev, 0));
+      if ((rc = ctx_mark_done(c))) return rc;
+    } else {
+      // same stream: the ensemble's marker covers the member
+      c->done_flag_shared = e->done_flag;
+      c->done_seq_shared = e->done_seq;
+    }
     c->vel_valid = true;
     c->vel_prec = prec;
     c->vel_form = form;
@@ -2551,6 +2659,26 @@ int lpm_ens_synchronize(lpm_bve_ensemble* e) {
   LPM_HIP(hipSetDevice(e->device));
   LPM_HIP(hipStreamSynchronize(e->stream));
   return ok();
+}
+
+namespace {
+int query_mark(const unsigned long long* flag, unsigned long long seq, int* done, const char* who) {
+  if (!done) return fail(LPM_ERR_ARG, std::string(who) + ": null output");
+  *done = mark_reached(flag, seq) ? 1 : 0;
+  return ok();
+}
+}  // namespace
+
+int lpm_ens_query(lpm_bve_ensemble* e, int* done) {
+  if (!e) return fail(LPM_ERR_ARG, "lpm_ens_query: null ensemble");
+  return query_mark(e->done_flag, e->done_seq, done, "lpm_ens_query");
+}
+
+int lpm_bve_query(lpm_bve_ctx* c, int* done) {
+  if (!c) return fail(LPM_ERR_ARG, "lpm_bve_query: null context");
+  const int rc = query_mark(c->done_flag, c->done_seq, done, "lpm_bve_query");
+  if (rc == LPM_OK && *done) *done = mark_reached(c->done_flag_shared, c->done_seq_shared) ? 1 : 0;
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -68,6 +68,18 @@ class Ensemble:
     def synchronize(self):
         call("lpm_ens_synchronize", self.h)
 
+    def stream(self) -> int:
+        """The ensemble's hipStream_t (for members created on it)."""
+        st = C.c_void_p()
+        call("lpm_ens_stream", self.h, C.byref(st))
+        return st.value
+
+    def done(self) -> bool:
+        """True when the steps queued by the ensemble have finished (non-blocking)."""
+        d = C.c_int()
+        call("lpm_ens_query", self.h, C.byref(d))
+        return bool(d.value)
+
 
 def autotune(ctx: Context, prec: int = PREC_FP32C, scheme: int = SCHEME_LAGR, reps: int = 2):
     """Time the candidate launch shapes on ctx's problem; the fastest becomes
@@ -134,10 +146,11 @@ def member_state(spec: MemberSpec):
     return m.vx, vz - mu, fx, fz - mu, farea
 
 
-def make_member(spec: MemberSpec, device: int = 0, on_device: bool = True) -> Context:
+def make_member(spec: MemberSpec, device: int = 0, on_device: bool = True, stream=None) -> Context:
     """A context holding the member's initial state, mean removed: built on the
-    device from the cached level mesh (default), or on the host and uploaded."""
-    c = Context(device)
+    device from the cached level mesh (default), or on the host and uploaded.
+    stream: queue the context's work there (e.g. its ensemble's stream)."""
+    c = Context(device, stream=stream)
     if on_device:
         c.set_state_level(spec.level, member_ic(spec), subtract_mean=True)
         return c

@@ -14,6 +14,9 @@ admission needs no host mesh, no host field evaluation and no upload.
 Each request's result is bit-identical to running it alone: a member keeps
 the decomposition it has alone and writes only its own partial slots.
 
+Requests below level 8 share the batched lane; level-8 requests run in lanes
+of their own (see Server), so short requests are not held to a level-8 step.
+
 Reported: completion latency (finish - arrival) p50 / p99 and aggregate
 pair interactions per second over the run.
 
@@ -95,18 +98,38 @@ def request_bytes(spec: MemberSpec) -> int:
 
 
 class Server:
+    """Continuous batching in lanes.  Requests below `large_level` share the
+    batched lane: one ensemble, one persistent pair kernel per evaluation over
+    all of their work, stepped one RK4 step at a time so that membership can
+    change between steps.  A request at `large_level` or above runs in a lane of
+    its own (its context's stream, all its steps queued at once), so a short
+    request never waits for a large request's step; the GPU interleaves the
+    lanes' workgroups."""
+
     def __init__(self, device: int = 0, budget_bytes: float = 200e9, max_active: int = 256, dt: float = 0.005,
                  omega: float = 2 * math.pi, prec: int = PREC_FP32C, keep_results: bool = False,
-                 log=None):
+                 log=None, large_level: int = 8):
         self.device = device
         self.budget = budget_bytes
         self.max_active = max_active
         self.dt, self.omega, self.prec = dt, omega, prec
         self.keep_results = keep_results
+        self.large_level = large_level
         self.ens = Ensemble(device)
-        self.active: list[Request] = []
-        self.used = 0
+        self.ens_stream = self.ens.stream()
+        self.batch: list[Request] = []  # members of the batched lane
+        self.large: list[Request] = []  # requests in lanes of their own
+        # retired contexts not yet destroyed: freeing device memory waits for
+        # the whole GPU, so it is done when nothing is in flight (or when an
+        # admission needs the memory)
+        self.retired: list[Context] = []
+        self.used = 0      # bytes of live requests
+        self.retired_bytes = 0
         self.log = log or (lambda s: None)
+
+    @property
+    def active(self) -> list[Request]:
+        return self.batch + self.large
 
     def warm(self, levels):
         """Build the device mesh cache for `levels` (once per process)."""
@@ -116,14 +139,21 @@ class Server:
             c.close()
 
     def _admit(self, r: Request, now: float):
+        large = r.spec.level >= self.large_level
         t = time.perf_counter()
-        r.ctx = make_member(r.spec, self.device)
+        # batched-lane members queue on the ensemble's stream; a large request
+        # gets a stream of its own
+        r.ctx = make_member(r.spec, self.device, stream=None if large else self.ens_stream)
         r.admit_s = time.perf_counter() - t
         r.bytes = request_bytes(r.spec)
         self.used += r.bytes
         r.admitted = now
-        self.ens.add(r.ctx)
-        self.active.append(r)
+        if large:
+            r.ctx.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=r.steps)
+            self.large.append(r)
+        else:
+            self.ens.add(r.ctx)
+            self.batch.append(r)
 
     def _retire(self, r: Request, now: float):
         if self.keep_results:
@@ -131,11 +161,21 @@ class Server:
         else:
             r.ctx.synchronize()
         r.finished = now
-        self.ens.remove(r.ctx)
-        r.ctx.close()
+        if r in self.batch:
+            self.ens.remove(r.ctx)
+            self.batch.remove(r)
+        else:
+            self.large.remove(r)
+        self.retired.append(r.ctx)
         r.ctx = None
         self.used -= r.bytes
-        self.active.remove(r)
+        self.retired_bytes += r.bytes
+
+    def _free_retired(self):
+        for c in self.retired:
+            c.close()
+        self.retired.clear()
+        self.retired_bytes = 0
 
     def run(self, requests: list[Request]) -> dict:
         for r in requests:
@@ -148,45 +188,76 @@ class Server:
         busy = 0.0
         pairs = 0.0
         steps = 0
-        while pending or self.active:
-            now = time.perf_counter() - t0
-            # admit arrived requests in arrival order while a slot and memory allow
-            while (pending and pending[0].arrival <= now and len(self.active) < self.max_active
-                   and self.used + request_bytes(pending[0].spec) <= self.budget):
-                self._admit(pending.pop(0), now)
-            if not self.active:
-                wait = pending[0].arrival - now
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
-                continue
-            ts = time.perf_counter()
-            self.ens.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=1)
-            self.ens.synchronize()
-            te = time.perf_counter()
-            busy += te - ts
-            steps += 1
-            now = te - t0
-            for r in list(self.active):
-                r.steps_done += 1
-                inc = pairs_per_step(r.spec.level)
-                r.pairs += inc
-                pairs += inc
-                if r.steps_done >= r.steps:
+        batch_inflight = False
+        last_log = last = t0
+        while pending or self.batch or self.large:
+            tn = time.perf_counter()
+            if batch_inflight or self.large:
+                busy += tn - last
+            last = tn
+            now = tn - t0
+            # completions
+            if batch_inflight and self.ens.done():
+                batch_inflight = False
+                steps += 1
+                for r in list(self.batch):
+                    r.steps_done += 1
+                    inc = pairs_per_step(r.spec.level)
+                    r.pairs += inc
+                    pairs += inc
+                    if r.steps_done >= r.steps:
+                        self._retire(r, now)
+                        done.append(r)
+            for r in list(self.large):
+                if r.ctx.done():
+                    r.steps_done = r.steps
+                    r.pairs = pairs_per_step(r.spec.level) * r.steps
+                    pairs += r.pairs
                     self._retire(r, now)
                     done.append(r)
+            if self.retired and not batch_inflight and not self.large:
+                self._free_retired()
+            # admission in arrival order while a slot and memory allow; the
+            # batched lane changes membership only between steps
+            while pending and pending[0].arrival <= now and len(self.active) < self.max_active:
+                need = request_bytes(pending[0].spec)
+                if self.used + self.retired_bytes + need > self.budget:
+                    if self.retired and self.used + need <= self.budget and not batch_inflight:
+                        self._free_retired()  # memory pressure: free now, even while a large lane runs
+                    else:
+                        break
+                if pending[0].spec.level < self.large_level and batch_inflight:
+                    break
+                self._admit(pending.pop(0), now)
+            if not batch_inflight and self.batch:
+                self.ens.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=1)
+                batch_inflight = True
+            if tn - last_log > 10.0:
+                last_log = tn
+                self.log(f"[serve] t={now:.0f}s done={len(done)} batched={len(self.batch)} large={len(self.large)} "
+                         f"pending={len(pending)} batched steps={steps} memory={self.used / 1e9:.1f} GB")
+            if not batch_inflight and not self.large:
+                if pending:
+                    time.sleep(min(max(pending[0].arrival - (time.perf_counter() - t0), 0.0), 0.05))
+            else:
+                time.sleep(0.0005)
         wall = time.perf_counter() - t0
+        self._free_retired()
         lat = np.array([r.latency for r in done])
         return {
             "requests": len(done),
             "wall_s": wall,
             "device_busy_s": busy,
-            "ensemble_steps": steps,
+            "batched_steps": steps,
+            "large_lane_requests": int(sum(1 for r in done if r.spec.level >= self.large_level)),
             "pairs": pairs,
             "pairs_per_s": pairs / wall,
             "pairs_per_busy_s": pairs / busy if busy else None,
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()),
+            "latency_p50_s_below_large": float(np.percentile([r.latency for r in done if r.spec.level < self.large_level]
+                                                             or [0.0], 50)),
             "admission_mean_s": float(np.mean([r.admit_s for r in done])),
             "admission_max_s": float(np.max([r.admit_s for r in done])),
             "levels": {int(l): int(sum(1 for r in done if r.spec.level == l)) for l in sorted({r.spec.level for r in done})},
@@ -225,11 +296,13 @@ def main(argv=None) -> int:
     probs = {l: p for l, p in LEVEL_PROBS.items() if l <= args.max_level}
     tot = sum(probs.values())
     probs = {l: p / tot for l, p in probs.items()}
+    log = lambda s: print(s, file=sys.stderr, flush=True)
     svc = measure_service_times(sorted(probs), steps=args.steps)
+    log(f"[serve] service seconds per request by level: {svc}")
     mean_service = sum(probs[l] * svc[l] for l in probs)
     rate = args.load / mean_service
     reqs = make_requests(args.requests, rate, seed=args.seed, level_probs=probs, steps=args.steps)
-    srv = Server(budget_bytes=args.budget_gb * 1e9, log=lambda s: print(s, file=sys.stderr, flush=True))
+    srv = Server(budget_bytes=args.budget_gb * 1e9, log=log)
     res = srv.run(reqs)
     srv.close()
     res.update({"arrival_rate_per_s": rate, "service_s_by_level": svc, "load": args.load,

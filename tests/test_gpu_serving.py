@@ -1,5 +1,6 @@
-"""Serving form: continuous batching admits requests as memory frees up, and
-every request's result is bit-identical to the same request run alone."""
+"""Serving form: continuous batching admits requests as memory frees up, every
+request's result is bit-identical to the same request run alone, and large
+requests run in lanes of their own without holding short ones back."""
 import math
 
 import numpy as np
@@ -16,10 +17,12 @@ def test_requests_match_alone_under_admission_pressure():
     reqs = S.make_requests(10, rate=200.0, seed=11, level_probs={2: 0.4, 3: 0.4, 4: 0.2}, steps=3)
     biggest = max(S.request_bytes(r.spec) for r in reqs)
     # room for about three requests at a time: later arrivals wait for retirements
-    srv = S.Server(budget_bytes=3.2 * biggest, keep_results=True)
+    # level-4 requests take the large lane, the rest share the batched lane
+    srv = S.Server(budget_bytes=3.2 * biggest, keep_results=True, large_level=4)
     res = srv.run(reqs)
     srv.close()
     assert res["requests"] == 10
+    assert 0 < res["large_lane_requests"] < 10
     assert res["latency_p99_s"] >= res["latency_p50_s"] > 0
     assert max(r.admitted - r.arrival for r in reqs) > 0  # some request queued
     for r in reqs:
@@ -37,3 +40,20 @@ def test_budget_rejects_oversized_request():
     with pytest.raises(ValueError):
         srv.run(reqs)
     srv.close()
+
+
+def test_large_lane_does_not_hold_short_requests():
+    """A level-7 request (about 3 s of device time) arrives first; level-2
+    requests arriving while it runs finish in a small fraction of its time,
+    because they step in the batched lane while the large one runs in its own."""
+    big = S.Request(rid=0, arrival=0.0, spec=E.MemberSpec(level=7, vortex_seed=5), steps=20)
+    small = [S.Request(rid=1 + k, arrival=0.2 + 0.05 * k, spec=E.MemberSpec(level=2, vortex_seed=10 + k), steps=20)
+             for k in range(8)]
+    srv = S.Server(large_level=7)
+    res = srv.run([big] + small)
+    srv.close()
+    assert res["requests"] == 9 and res["large_lane_requests"] == 1
+    worst_small = max(r.latency for r in small)
+    print(f"large request {big.latency:.2f} s, slowest short request {worst_small:.3f} s")
+    assert big.finished > max(r.finished for r in small)  # the short ones overtook it
+    assert worst_small < 0.25 * big.latency
