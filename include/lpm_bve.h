@@ -91,6 +91,12 @@ int lpm_ic_eval(const double* xyz, int64_t n, double* zeta, double sbr, int rh_n
  * standard normals (used to draw ensemble and request parameters). */
 int lpm_rng_draws(uint64_t seed, int64_t nuniform, double* uniform, int64_t nnormal, double* normal);
 
+/* The Poisson problem of SpherePoisson (src/LpmSpherePoisson.hpp:60-80):
+ * f = Y_5^4(x) = 30 cos(4 lon) z (z^2-1)^2 (exact stream function f/30) and
+ * its velocity RH54Velocity (src/LpmRossbyWaves.hpp:39-53) at n points.
+ * f [n] and u [n][3] may each be NULL. */
+int lpm_poisson_y54(const double* xyz, int64_t n, double* f, double* u);
+
 /* Area-weighted mean of zeta over unmasked faces, summed in index order. */
 int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, double* mean);
 

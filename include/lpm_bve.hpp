@@ -19,7 +19,9 @@
 #ifndef LPM_BVE_HPP
 #define LPM_BVE_HPP
 
+#include <algorithm>
 #include <cmath>
+#include <iostream>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -265,6 +267,122 @@ class BVERK4 {
     check(lpm_bve_advance(s->context(), dt, Omega, scheme, prec, nsteps, use_graph ? 1 : 0), "lpm_bve_advance");
     s->t += nsteps * dt;
   }
+};
+
+// SpherePoisson<Seed> (src/LpmSpherePoisson.hpp:359-520): -lap(psi) = f on
+// the sphere with f = Y_5^4 (exact psi = f/30, exact velocity RH54Velocity),
+// solved by the same direct sums as the BVE path (stream function and
+// velocity at vertices and faces, FP64, literal 1 - x.y, on the GPU), with the
+// reference's L-infinity errors over vertices and active faces.
+template <typename SeedType = IcosTriSphereSeed>
+class SpherePoisson {
+ public:
+  std::vector<Real> fverts, psiverts, psiexactverts, ffaces, psifaces, psiexactfaces;
+  std::vector<Real> everts, efaces;
+  std::vector<Real> uverts, ufaces, uvertsexact, ufacesexact, uerrverts, uerrfaces;
+  Real linf_verts = 0, linf_faces = 0, linf_uverts = 0, linf_ufaces = 0;
+  std::vector<Real> vert_x, face_x, face_area;
+  std::vector<uint8_t> face_mask;
+
+  SpherePoisson(Index nmaxverts, Index nmaxedges, Index nmaxfaces, int device = 0)
+      : nmaxv_(nmaxverts), nmaxf_(nmaxfaces), device_(device) {
+    (void)nmaxedges;
+  }
+
+  void treeInit(int depth, const MeshSeed<SeedType>& seed) {
+    Index nv, ne, nf;
+    seed.setMaxAllocations(nv, ne, nf, depth);
+    if (nv > nmaxv_ || nf > nmaxf_) throw std::runtime_error("treeInit: not enough memory");
+    vert_x.assign(3 * nv, 0);
+    face_x.assign(3 * nf, 0);
+    face_area.assign(nf, 0);
+    face_mask.assign(nf, 0);
+    int64_t counts[4];
+    check(lpm_icos_tree_init(depth, seed.path.empty() ? nullptr : seed.path.c_str(), vert_x.data(), nullptr,
+                             face_x.data(), nullptr, face_area.data(), face_mask.data(), nullptr, nullptr, counts),
+          "treeInit");
+    nverts_ = counts[0];
+    nfaces_ = counts[2];
+    nleaves_ = counts[3];
+  }
+
+  Index nvertsHost() const { return nverts_; }
+  Index nfacesHost() const { return nfaces_; }
+
+  // average mesh size (src/LpmSpherePoisson.hpp:410-415)
+  Real meshSize() const { return std::sqrt(4 * PI / nleaves_); }
+
+  // right-hand side and exact solution at every particle (Init, :60-84)
+  void init() {
+    fverts.assign(nverts_, 0);
+    ffaces.assign(nfaces_, 0);
+    uvertsexact.assign(3 * nverts_, 0);
+    ufacesexact.assign(3 * nfaces_, 0);
+    check(lpm_poisson_y54(vert_x.data(), nverts_, fverts.data(), uvertsexact.data()), "lpm_poisson_y54");
+    check(lpm_poisson_y54(face_x.data(), nfaces_, ffaces.data(), ufacesexact.data()), "lpm_poisson_y54");
+    psiexactverts.resize(nverts_);
+    psiexactfaces.resize(nfaces_);
+    for (Index i = 0; i < nverts_; ++i) psiexactverts[i] = fverts[i] / 30.0;
+    for (Index i = 0; i < nfaces_; ++i) psiexactfaces[i] = ffaces[i] / 30.0;
+  }
+
+  // Stream function and velocity at vertices and faces, then the errors
+  // (:428-475); prints the reference's lines.
+  void solve(const int& nthreads = 0) {
+    (void)nthreads;
+    lpm_bve_ctx* ctx = nullptr;
+    check(lpm_bve_create(device_, &ctx), "lpm_bve_create");
+    psiverts.assign(nverts_, 0);
+    psifaces.assign(nfaces_, 0);
+    uverts.assign(3 * nverts_, 0);
+    ufaces.assign(3 * nfaces_, 0);
+    int rc = lpm_bve_set_state(ctx, nverts_, vert_x.data(), fverts.data(), nfaces_, face_x.data(), ffaces.data(),
+                               face_area.data(), face_mask.data());
+    if (rc == LPM_OK) rc = lpm_bve_solve(ctx, LPM_PREC_FP64, LPM_SCHEME_REF, 1);
+    if (rc == LPM_OK)
+      rc = lpm_bve_get_state(ctx, nullptr, nullptr, uverts.data(), psiverts.data(), nullptr, nullptr, ufaces.data(),
+                             psifaces.data());
+    lpm_bve_destroy(ctx);
+    check(rc, "SpherePoisson::solve");
+    everts.assign(nverts_, 0);
+    efaces.assign(nfaces_, 0);
+    uerrverts.assign(3 * nverts_, 0);
+    uerrfaces.assign(3 * nfaces_, 0);
+    linf_verts = linf_faces = linf_uverts = linf_ufaces = 0;
+    for (Index i = 0; i < nverts_; ++i) {
+      everts[i] = std::abs(psiverts[i] - psiexactverts[i]);
+      linf_verts = std::max(linf_verts, everts[i]);
+      Real e2 = 0;
+      for (int k = 0; k < 3; ++k) {
+        uerrverts[3 * i + k] = std::abs(uverts[3 * i + k] - uvertsexact[3 * i + k]);
+        e2 += uerrverts[3 * i + k] * uerrverts[3 * i + k];
+      }
+      linf_uverts = std::max(linf_uverts, e2);
+    }
+    for (Index i = 0; i < nfaces_; ++i) {
+      if (face_mask[i]) continue;  // divided faces: error 0
+      efaces[i] = std::abs(psifaces[i] - psiexactfaces[i]);
+      linf_faces = std::max(linf_faces, efaces[i]);
+      Real e2 = 0;
+      for (int k = 0; k < 3; ++k) {
+        uerrfaces[3 * i + k] = std::abs(ufaces[3 * i + k] - ufacesexact[3 * i + k]);
+        e2 += uerrfaces[3 * i + k] * uerrfaces[3 * i + k];
+      }
+      linf_ufaces = std::max(linf_ufaces, e2);
+    }
+    std::cout << meshSize() * (180.0 / PI) << ":\n"
+              << "\tpotential: linf_verts = " << linf_verts << ", linf_faces = " << linf_faces << '\n'
+              << "\tvelocity:  linf_verts = " << std::sqrt(linf_uverts) << ", linf_faces = " << std::sqrt(linf_ufaces)
+              << '\n';
+  }
+
+  void updateHost() const {}
+  void updateDevice() const {}
+
+ private:
+  Index nmaxv_, nmaxf_;
+  int device_;
+  Index nverts_ = 0, nfaces_ = 0, nleaves_ = 0;
 };
 
 }  // namespace Lpm

@@ -5,4 +5,5 @@ ABI of include/lpm_bve.h); this package is its Python front end.
 """
 from ._lib import PREC_FP32C, PREC_FP64, SCHEME_LAGR, SCHEME_REF, LpmError, lib  # noqa: F401
 from .bve import (BVERK4, BVESphere, Context, IcosTriSphereSeed, MeshSeed, RossbyHaurwitzWave,  # noqa: F401
-                  SolidBodyRotation, TreeMesh, VorticityInitialCondition, area_mean, gauss_params)
+                  SolidBodyRotation, SpherePoisson, TreeMesh, VorticityInitialCondition, area_mean, gauss_params,
+                  poisson_y54)

@@ -382,3 +382,66 @@ class BVERK4:
             raise ValueError("advance_timestep: init(nverts, nfaces) does not match the sphere")
         sphere.ctx.advance(self.dt, self.Omega, self.scheme, self.prec, nsteps, self.use_graph)
         sphere.t += nsteps * self.dt
+
+
+# ---------------------------------------------------------------------------
+# SpherePoisson (src/LpmSpherePoisson.hpp:359-520)
+# ---------------------------------------------------------------------------
+def poisson_y54(xyz):
+    """f = Y_5^4 and the velocity RH54Velocity at points xyz [n][3]."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    f = np.zeros(xyz.shape[0])
+    u = np.zeros_like(xyz)
+    call("lpm_poisson_y54", xyz, xyz.shape[0], _lib.ptr(f), _lib.ptr(u))
+    return f, u
+
+
+class SpherePoisson:
+    """-lap(psi) = f with f = Y_5^4 on the sphere (exact psi = f/30, exact
+    velocity RH54Velocity), solved by the direct sums on the GPU (FP64, literal
+    1 - x.y); the reference's L-infinity errors over vertices and active faces."""
+
+    def __init__(self, nmaxverts: int, nmaxedges: int, nmaxfaces: int, device: int = 0):
+        self.nmax = (nmaxverts, nmaxedges, nmaxfaces)
+        self.device = device
+        self.linf_verts = self.linf_faces = self.linf_uverts = self.linf_ufaces = 0.0
+
+    def treeInit(self, depth: int, seed: MeshSeed | None = None):
+        self.mesh = TreeMesh(depth, seed)
+        if self.mesh.nverts > self.nmax[0] or self.mesh.nfaces > self.nmax[2]:
+            raise ValueError("treeInit: not enough memory")
+
+    def meshSize(self) -> float:
+        return math.sqrt(4 * PI / self.mesh.nleaves)
+
+    def init(self):
+        m = self.mesh
+        self.fverts, self.uvertsexact = poisson_y54(m.vx)
+        self.ffaces, self.ufacesexact = poisson_y54(m.fx)
+        self.psiexactverts = self.fverts / 30.0
+        self.psiexactfaces = self.ffaces / 30.0
+
+    def solve(self, nthreads: int = 0, quiet: bool = False):
+        m = self.mesh
+        c = Context(self.device)
+        try:
+            c.set_state(m.vx, self.fverts, m.fx, self.ffaces, m.farea, m.fmask)
+            c.solve(PREC_FP64, SCHEME_REF, want_psi=True)
+            s = c.get_state()
+        finally:
+            c.close()
+        self.psiverts, self.psifaces, self.uverts, self.ufaces = s["vpsi"], s["fpsi"], s["vu"], s["fu"]
+        lv = m.fmask == 0
+        self.everts = np.abs(self.psiverts - self.psiexactverts)
+        self.efaces = np.where(lv, np.abs(self.psifaces - self.psiexactfaces), 0.0)
+        self.uerrverts = np.abs(self.uverts - self.uvertsexact)
+        self.uerrfaces = np.where(lv[:, None], np.abs(self.ufaces - self.ufacesexact), 0.0)
+        self.linf_verts = float(self.everts.max())
+        self.linf_faces = float(self.efaces.max())
+        self.linf_uverts = float(np.sum(self.uerrverts ** 2, axis=1).max())
+        self.linf_ufaces = float(np.sum(self.uerrfaces ** 2, axis=1).max())
+        if not quiet:
+            print(f"{self.meshSize() * 180.0 / PI:g}:\n"
+                  f"\tpotential: linf_verts = {self.linf_verts:g}, linf_faces = {self.linf_faces:g}\n"
+                  f"\tvelocity:  linf_verts = {math.sqrt(self.linf_uverts):g}, "
+                  f"linf_faces = {math.sqrt(self.linf_ufaces):g}")

@@ -115,6 +115,20 @@ int lpm_ic_eval(const double* xyz, int64_t n, double* zeta, double sbr, int rh_n
   return ok();
 }
 
+int lpm_poisson_y54(const double* xyz, int64_t n, double* f, double* u) {
+  if (n < 0 || (n && !xyz)) return fail(LPM_ERR_ARG, "lpm_poisson_y54: bad arguments");
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const Vec3 p{{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}};
+    if (f) f[i] = sph_harm54(p);
+    if (u) {
+      const Vec3 v = rh54_velocity(p);
+      for (int k = 0; k < 3; ++k) u[3 * i + k] = v[k];
+    }
+  }
+  return ok();
+}
+
 int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, double* mean) {
   double num = 0.0, den = 0.0;
   for (int64_t i = 0; i < nf; ++i) {

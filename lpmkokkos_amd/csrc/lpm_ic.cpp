@@ -70,4 +70,21 @@ void CompositeVorticity::draw_vortices(uint64_t seed, int nvort, double amp_scal
   sigma = sigma_;
 }
 
+double sph_harm54(const Vec3& x) {
+  const double lon = sphere::longitude(x);
+  const double z = x[2];
+  const double p54 = z * (z * z - 1.0) * (z * z - 1.0);
+  return 30 * std::cos(4 * lon) * p54;
+}
+
+Vec3 rh54_velocity(const Vec3& x) {
+  const double lat = sphere::latitude(x), lon = sphere::longitude(x);
+  const double cl = std::cos(lat), sl = std::sin(lat), cn = std::cos(lon), sn = std::sin(lon);
+  const double cl3 = cl * cl * cl;
+  // zonal and meridional components, then the Cartesian velocity
+  const double us = 0.5 * std::cos(4 * lon) * cl3 * (5 * std::cos(2 * lat) - 3);
+  const double vs = 4 * cl3 * sl * std::sin(4 * lon);
+  return Vec3{{-us * sn - vs * sl * cn, us * cn - vs * sl * sn, vs * cl}};
+}
+
 }  // namespace lpm

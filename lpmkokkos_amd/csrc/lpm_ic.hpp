@@ -69,4 +69,11 @@ struct RossbyHaurwitzWave : public CompositeVorticity {
   std::string name() const override { return "rossby_haurwitz"; }
 };
 
+// The Poisson problem of SpherePoisson (src/LpmSpherePoisson.hpp:60-80,
+// src/LpmRossbyWaves.hpp:18-53): right-hand side f = Y_5^4 =
+// 30 cos(4 lon) P_5^4(z) with P_5^4(z) = z (z^2 - 1)^2, exact stream function
+// f / 30, and the velocity it induces (RH54Velocity).
+double sph_harm54(const Vec3& x);
+Vec3 rh54_velocity(const Vec3& x);
+
 }  // namespace lpm

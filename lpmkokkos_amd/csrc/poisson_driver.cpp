@@ -1,0 +1,35 @@
+// The reference's SpherePoisson test program (tests/LpmSpherePoisson.cpp:9-55)
+// on this library, icosahedral seed: for each tree depth, build the mesh,
+// set the Y_5^4 right-hand side, solve on the GPU and print the mesh size and
+// L-infinity errors in the reference's format.
+//
+//     poisson_driver [max_depth]     (depths 0 .. max_depth, default 3)
+#include <cstdlib>
+#include <iostream>
+
+#include "lpm_bve.hpp"
+
+using namespace Lpm;
+
+int main(int argc, char* argv[]) {
+  const int max_depth = argc > 1 ? std::atoi(argv[1]) : 3;
+  try {
+    for (int tree_depth = 0; tree_depth <= max_depth; ++tree_depth) {
+      Index nmaxverts, nmaxedges, nmaxfaces;
+      MeshSeed<IcosTriSphereSeed> triseed;
+      triseed.setMaxAllocations(nmaxverts, nmaxedges, nmaxfaces, tree_depth);
+      SpherePoisson<IcosTriSphereSeed> ic(nmaxverts, nmaxedges, nmaxfaces);
+      ic.treeInit(tree_depth, triseed);
+      ic.updateDevice();
+      ic.init();
+      std::cout << "icostri: ";
+      ic.solve();
+      ic.updateHost();
+    }
+  } catch (const std::exception& e) {
+    std::cerr << "poisson_driver: " << e.what() << std::endl;
+    return 1;
+  }
+  std::cout << "tests pass." << std::endl;
+  return 0;
+}

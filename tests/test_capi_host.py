@@ -110,3 +110,16 @@ def test_request_generator():
     assert np.all(np.diff(arr) > 0) and abs(np.mean(np.diff(arr)) - 2.0) < 0.2
     assert {r.spec.rh_n for r in reqs} == {2, 3, 4, 5, 6}
     assert min(r.spec.nvort for r in reqs) == 4 and max(r.spec.nvort for r in reqs) == 16
+
+
+def test_poisson_y54_matches_oracle(oracle_mod):
+    """Host Y_5^4 right-hand side and RH54 velocity (src/LpmSpherePoisson.hpp:60-80,
+    src/LpmRossbyWaves.hpp:18-53) against the oracle's restatement: velocity bit
+    for bit; f to 2 ulp (the oracle's compiler may merge cos(4 lon) with the
+    velocity's sin(4 lon) into one sincos call, which rounds differently)."""
+    m = L.TreeMesh(3)
+    for x in (m.vx, m.fx):
+        f, u = L.poisson_y54(x)
+        of, ou = oracle_mod.sphharm54(x)
+        assert np.all(np.abs(f - of) <= 2 * np.spacing(np.abs(of)))
+        assert np.array_equal(u, ou)
