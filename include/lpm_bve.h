@@ -49,6 +49,14 @@ typedef enum {
   LPM_SCHEME_LAGR = 1
 } lpm_scheme;
 
+/* VTK polydata layouts of lpm_bve_write_vtk. */
+typedef enum {
+  LPM_VTK_LEGACY_ASCII = 0,  /* BVESphere::outputVtk (src/LpmBVESphere.cpp:90-113): legacy .vtk */
+  LPM_VTK_LEGACY_BINARY = 1,
+  LPM_VTK_XML_ASCII = 2,     /* Polymesh2dVtkInterface::write + addFieldsToVtk: .vtp */
+  LPM_VTK_XML_BINARY = 3     /* .vtp, inline base64 */
+} lpm_vtk_format;
+
 const char* lpm_last_error(void);
 int lpm_version(void); /* major*10000 + minor*100 + patch */
 int lpm_device_count(int* count);
@@ -69,6 +77,21 @@ int lpm_icos_alloc(int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces)
  * counts (non-NULL) receives {nverts, nedges, nfaces, nleaves}. */
 int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vlag, double* fx, double* flag,
                        double* farea, uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts);
+
+/* NcWriter::writePolymesh (src/LpmNetCDF_Impl.hpp:46-218) of the treeInit mesh
+ * (as tests/LpmNetCDFTest.cpp:95-104 writes icostri_sphere.nc): dimensions
+ * ndim, nverts, nedges, nfaces, nfaceverts, two, four; coordinates, edge and
+ * face topology and tree, areas and mask; attributes MeshSeed, FaceKind, Geom,
+ * baseTreeDepth.  A netCDF classic (64-bit offset) file. */
+int lpm_mesh_write_nc(int level, const char* seed_path, const char* path);
+
+/* PolyMeshReader (src/LpmNetCDF.cpp:14-240): the particle arrays of a mesh
+ * file written by lpm_mesh_write_nc or lpm_bve_checkpoint, in the layout of
+ * lpm_icos_tree_init.  Call with NULL arrays to get counts {nverts, nedges,
+ * nfaces, nleaves} (nedges -1 when the file holds no edges); fverts and
+ * fparent need a file with topology. */
+int lpm_mesh_read_nc(const char* path, double* vx, double* vlag, double* fx, double* flag, double* farea,
+                     uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts);
 
 /* ------------------------------------------------------------------ */
 /* Vorticity initial conditions (host), the                           */
@@ -191,6 +214,46 @@ int lpm_bve_advance(lpm_bve_ctx* ctx, double dt, double omega, int scheme, int p
  * Blocks until the context's stream has drained. */
 int lpm_bve_get_state(lpm_bve_ctx* ctx, double* vx, double* vzeta, double* vu, double* vpsi, double* fx,
                       double* fzeta, double* fu, double* fpsi);
+
+/* Checkpoint: the state in a netCDF file that lpm_bve_restore resumes from
+ * bit for bit.  The mesh variables of NcWriter::writePolymesh (all of them
+ * when the tree is known: `level` >= 0 names it, -1 takes a level-built
+ * state's own; else positions, areas and mask only), the BVESphere fields
+ * under their view labels (relVortVerts, absVortVerts, streamFnVerts,
+ * velocityVerts, and *Faces; src/LpmBVESphere.cpp:13-20), and attributes
+ * time, Omega, velocity_prec, velocity_form, streamfn_valid, absvort_valid,
+ * absvort_omega.  The absolute vorticity is that of the first step at rate
+ * `omega` (computed now if no step has run at that rate). */
+int lpm_bve_checkpoint(lpm_bve_ctx* ctx, const char* path, int level, double time, double omega);
+
+/* Resume from a checkpoint: the state, its velocity (with the precision and
+ * form that produced it), psi and absolute vorticity, so that the next steps
+ * are those the checkpointed context would have taken.  time and omega
+ * (non-NULL) receive the stored values. */
+int lpm_bve_restore(lpm_bve_ctx* ctx, const char* path, double* time, double* omega);
+
+/* VTK polydata of the state on the leaf faces of its tree (`level` as for
+ * lpm_bve_checkpoint; required unless the state was built by level).  Point
+ * data relVort, absVort, streamFn, velocity; cell data area, relVort, absVort,
+ * streamFn, velocity (legacy names; the XML layouts use addFieldsToVtk's
+ * relvort, absvort, stream_fn, velocity).  streamFn is zero unless the last
+ * solve computed psi. */
+int lpm_bve_write_vtk(lpm_bve_ctx* ctx, const char* path, int level, double omega, int format);
+
+/* A caller's array for lpm_bve_write_vtk_arrays: at vertices [nverts][ncomp]
+ * or at faces [nfaces][ncomp] in tree order (leaf rows are written). */
+typedef struct {
+  const char* name; /* no blanks or XML markup */
+  int ncomp;        /* 1..9 */
+  int on_faces;     /* 0: point data, 1: cell data */
+  const double* data;
+} lpm_vtk_array;
+
+/* lpm_bve_write_vtk with `narrays` more arrays after the state's own, as the
+ * reference's BVE driver adds its error fields with addVectorPointData /
+ * addVectorCellData (tests/LpmBVETest.cpp:98-106, 228-238). */
+int lpm_bve_write_vtk_arrays(lpm_bve_ctx* ctx, const char* path, int level, double omega, int format,
+                             const lpm_vtk_array* arrays, int narrays);
 
 /* Total circulation sum over unmasked faces of zeta * area (FP64, device). */
 int lpm_bve_circulation(lpm_bve_ctx* ctx, double* circ);

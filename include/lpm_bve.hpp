@@ -43,6 +43,7 @@ inline void check(int rc, const char* what) {
 struct IcosTriSphereSeed {
   static constexpr int nverts = 12, nfaces = 20, nedges = 30, nfaceverts = 3;
   static std::string idString() { return "IcosTriSphereSeed"; }
+  static std::string faceStr() { return "tri"; }
 };
 
 template <typename SeedType>
@@ -176,6 +177,7 @@ class BVESphere {
     nverts_ = counts[0];
     nfaces_ = counts[2];
     nleaves_ = counts[3];
+    depth_ = depth;
   }
 
   Index nvertsHost() const { return nverts_; }
@@ -222,6 +224,51 @@ class BVESphere {
           "lpm_bve_get_state");
   }
 
+  // One more field for outputVtk: [nverts][ncomp] at vertices, or
+  // [nfaces][ncomp] at faces in tree order.
+  struct VtkArray {
+    std::string name;
+    int ncomp = 1;
+    bool on_faces = false;
+    std::vector<Real> data;
+  };
+
+  // BVESphere::outputVtk (src/LpmBVESphere.cpp:90-113): legacy VTK polydata of
+  // the leaf faces; a name ending in .vtp writes the XML layout of
+  // Polymesh2dVtkInterface::write with addFieldsToVtk.  `extra` fields follow
+  // the state's own.
+  void outputVtk(const std::string& fname, const std::vector<VtkArray>& extra = {}) {
+    const bool vtp = fname.size() > 4 && fname.compare(fname.size() - 4, 4, ".vtp") == 0;
+    std::vector<lpm_vtk_array> a;
+    for (const VtkArray& x : extra) a.push_back({x.name.c_str(), x.ncomp, x.on_faces ? 1 : 0, x.data.data()});
+    check(lpm_bve_write_vtk_arrays(ctx_, fname.c_str(), depth_, Omega, vtp ? LPM_VTK_XML_BINARY : LPM_VTK_LEGACY_ASCII,
+                                   a.data(), (int)a.size()),
+          "outputVtk");
+  }
+
+  // The state, time and rotation rate in one netCDF file (the mesh variables of
+  // NcWriter::writePolymesh and the fields under the view labels); restore()
+  // continues bit for bit.
+  void checkpoint(const std::string& fname) {
+    check(lpm_bve_checkpoint(ctx_, fname.c_str(), depth_, t, Omega), "checkpoint");
+  }
+  void restore(const std::string& fname) {
+    check(lpm_bve_restore(ctx_, fname.c_str(), &t, &Omega), "restore");
+    omg_set_ = true;
+    int64_t sz[4];
+    check(lpm_bve_sizes(ctx_, sz), "lpm_bve_sizes");
+    nverts_ = sz[0];
+    nfaces_ = sz[1];
+    nleaves_ = sz[2];
+    depth_ = -1;
+    for (int lev = 0; lev <= 12; ++lev)
+      if (2 + 10 * (int64_t(1) << (2 * lev)) == nverts_) depth_ = lev;
+    vert_x.assign(3 * nverts_, 0);
+    face_x.assign(3 * nfaces_, 0);
+    rel_vort_verts.assign(nverts_, 0);
+    rel_vort_faces.assign(nfaces_, 0);
+  }
+
   Real avg_mesh_size_radians() const { return std::sqrt(4 * PI / nfaces_); }
   Real avg_mesh_size_degrees() const { return avg_mesh_size_radians() * 180.0 / PI; }
   Real circulation() {
@@ -235,6 +282,7 @@ class BVESphere {
   lpm_bve_ctx* ctx_ = nullptr;
   Index nmaxv_, nmaxf_;
   Index nverts_ = 0, nfaces_ = 0, nleaves_ = 0;
+  int depth_ = -1;
   bool omg_set_ = false;
 };
 

@@ -18,6 +18,7 @@ HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "lpm_bve.h")
 LPM_OK = 0
 PREC_FP64, PREC_FP32C = 0, 1
 SCHEME_REF, SCHEME_LAGR = 0, 1
+VTK_LEGACY_ASCII, VTK_LEGACY_BINARY, VTK_XML_ASCII, VTK_XML_BINARY = 0, 1, 2, 3
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _dpn = C.c_void_p  # nullable double*
@@ -34,6 +35,11 @@ class IcParams(C.Structure):
                 ("subtract_mean", C.c_int)]
 
 
+class VtkArray(C.Structure):
+    """lpm_vtk_array of include/lpm_bve.h."""
+    _fields_ = [("name", C.c_char_p), ("ncomp", C.c_int), ("on_faces", C.c_int), ("data", C.c_void_p)]
+
+
 # (name, argtypes) for every entry point of include/lpm_bve.h
 SIGNATURES = {
     "lpm_last_error": [],
@@ -46,6 +52,8 @@ SIGNATURES = {
                     C.c_double],
     "lpm_area_mean": [_dp, _dp, _bp, C.c_int64, C.POINTER(C.c_double)],
     "lpm_poisson_y54": [_dp, C.c_int64, _dpn, _dpn],
+    "lpm_mesh_write_nc": [C.c_int, C.c_char_p, C.c_char_p],
+    "lpm_mesh_read_nc": [C.c_char_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
     "lpm_rng_draws": [C.c_uint64, C.c_int64, _dpn, C.c_int64, _dpn],
     "lpm_bve_vertex_velocity": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int64, C.c_void_p],
@@ -63,6 +71,10 @@ SIGNATURES = {
     "lpm_bve_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int],
     "lpm_bve_advance": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
     "lpm_bve_get_state": [C.c_void_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
+    "lpm_bve_checkpoint": [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_double],
+    "lpm_bve_restore": [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p],
+    "lpm_bve_write_vtk": [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_int],
+    "lpm_bve_write_vtk_arrays": [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_int],
     "lpm_bve_circulation": [C.c_void_p, C.POINTER(C.c_double)],
     "lpm_bve_synchronize": [C.c_void_p],
     "lpm_bve_query": [C.c_void_p, C.POINTER(C.c_int)],

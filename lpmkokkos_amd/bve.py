@@ -94,6 +94,31 @@ class TreeMesh:
     def leaf_index(self):
         return np.nonzero(self.fmask == 0)[0]
 
+    @classmethod
+    def from_nc(cls, path: str) -> "TreeMesh":
+        """PolyMesh2d(const PolyMeshReader&) (src/LpmPolyMesh2d.cpp:20-28): the
+        arrays of a mesh file (write_mesh_nc or a checkpoint with topology)."""
+        counts = np.zeros(4, np.int64)
+        call("lpm_mesh_read_nc", path.encode(), *([None] * 8), _lib.ptr(counts))
+        nv, ne, nf = (int(c) for c in counts[:3])
+        m = cls.__new__(cls)
+        m.vx, m.vlag = np.zeros((nv, 3)), np.zeros((nv, 3))
+        m.fx, m.flag = np.zeros((nf, 3)), np.zeros((nf, 3))
+        m.farea, m.fmask = np.zeros(nf), np.zeros(nf, np.uint8)
+        m.fverts, m.fparent = np.zeros((nf, 3), np.int32), np.zeros(nf, np.int32)
+        call("lpm_mesh_read_nc", path.encode(),
+             *(_lib.ptr(a) for a in (m.vx, m.vlag, m.fx, m.flag, m.farea, m.fmask, m.fverts, m.fparent, counts)))
+        m.nverts, m.nedges, m.nfaces, m.nleaves = (int(c) for c in counts)
+        m.level = next((lev for lev in range(13) if IcosTriSphereSeed.nVerticesAtTreeLevel(lev) == nv), -1)
+        return m
+
+
+def write_mesh_nc(level: int, path: str, seed: MeshSeed | None = None):
+    """NcWriter::writePolymesh (src/LpmNetCDF_Impl.hpp:46-218) of the level-`level`
+    treeInit mesh, as a netCDF classic (64-bit offset) file."""
+    seed = seed or MeshSeed()
+    call("lpm_mesh_write_nc", int(level), seed.path.encode() if seed.path else None, path.encode())
+
 
 # ---------------------------------------------------------------------------
 # initial conditions: VorticityInitialCondition::eval (src/LpmVorticityGallery.hpp:13-40)
@@ -258,6 +283,41 @@ class Context:
         call("lpm_bve_circulation", self.h, C.byref(c))
         return c.value
 
+    def checkpoint(self, path: str, time: float = 0.0, omega: float = 0.0, level: int = -1):
+        """Write the state to a netCDF file that restore() resumes from bit for
+        bit (mesh variables, BVESphere fields, time and rotation rate).  level:
+        the tree the state was built on, for the full mesh topology (-1: the
+        state's own level when built by level, else particles only)."""
+        call("lpm_bve_checkpoint", self.h, path.encode(), int(level), float(time), float(omega))
+
+    def restore(self, path: str):
+        """Load a checkpoint; returns its (time, omega)."""
+        t, om = C.c_double(), C.c_double()
+        call("lpm_bve_restore", self.h, path.encode(), C.byref(t), C.byref(om))
+        sizes = self.sizes()
+        self.nv, self.nf = sizes[0], sizes[1]
+        return t.value, om.value
+
+    def write_vtk(self, path: str, omega: float = 0.0, level: int = -1, fmt: int = _lib.VTK_LEGACY_ASCII,
+                  arrays=None):
+        """VTK polydata of the state on its tree's leaf faces (fmt: VTK_LEGACY_ASCII,
+        VTK_LEGACY_BINARY, VTK_XML_ASCII, VTK_XML_BINARY).  arrays: more fields as
+        (name, on_faces, values) with values [nverts] / [nverts][k] at vertices or
+        [nfaces] / [nfaces][k] at faces (tree order)."""
+        arrays = arrays or []
+        keep = []
+        items = (_lib.VtkArray * max(1, len(arrays)))()
+        for i, (name, on_faces, values) in enumerate(arrays):
+            a = np.ascontiguousarray(values, np.float64)
+            n = self.nf if on_faces else self.nv
+            if a.shape[0] != n:
+                raise ValueError(f"array {name}: {a.shape[0]} rows, expected {n}")
+            keep.append(a)
+            items[i] = _lib.VtkArray(name.encode(), 1 if a.ndim == 1 else int(a.shape[1]), int(bool(on_faces)),
+                                     a.ctypes.data)
+        call("lpm_bve_write_vtk_arrays", self.h, path.encode(), int(level), float(omega), int(fmt), items,
+             len(arrays))
+
     def get_state(self):
         nv, nf = self.nv, self.nf
         out = dict(vx=np.zeros((nv, 3)), vzeta=np.zeros(nv), vu=np.zeros((nv, 3)), vpsi=np.zeros(nv),
@@ -345,6 +405,27 @@ class BVESphere:
         s = self.ctx.get_state()
         self.host.update(s)
         return s
+
+    def outputVtk(self, fname: str, arrays=None):
+        """BVESphere::outputVtk (src/LpmBVESphere.cpp:90-113): legacy VTK polydata
+        of the leaf faces; a .vtp name writes the XML layout of
+        Polymesh2dVtkInterface::write with addFieldsToVtk instead.  arrays: more
+        fields, as Context.write_vtk takes them."""
+        fmt = _lib.VTK_XML_BINARY if fname.endswith(".vtp") else _lib.VTK_LEGACY_ASCII
+        self.ctx.write_vtk(fname, self.Omega, self.mesh.level, fmt, arrays)
+
+    def checkpoint(self, fname: str):
+        """The mesh, fields, time and rotation rate in one netCDF file."""
+        self.ctx.checkpoint(fname, self.t, self.Omega, self.mesh.level)
+
+    def restore(self, fname: str):
+        """Resume from checkpoint(); the tree is rebuilt from its level."""
+        self.t, self.Omega = self.ctx.restore(fname)
+        self._omg_set = True
+        nv = self.ctx.nv
+        level = next(lev for lev in range(13) if IcosTriSphereSeed.nVerticesAtTreeLevel(lev) == nv)
+        if self.mesh is None or self.mesh.level != level:
+            self.mesh = TreeMesh(level)
 
     def avg_mesh_size_radians(self) -> float:
         return math.sqrt(4 * PI / self.nfacesHost())

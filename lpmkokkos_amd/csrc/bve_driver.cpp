@@ -2,7 +2,14 @@
 // on the MI355X library, through the C++ interface of include/lpm_bve.hpp.
 //
 //   bve_driver [-d depth] [-dt dt] [-tf tfinal] [-ic sbr|rh] [-scheme ref|lagr]
-//              [-prec fp64|fp32c] [-f output_interval]
+//              [-prec fp64|fp32c] [-f output_interval] [-o case_name]
+//
+// With -o, the state is written as XML VTK polydata at t = 0 and every
+// output_interval steps, named as the reference's driver names its files
+// (case_name + "tri" + depth + "_dt" + dt + "_" + 4-digit step + ".vtp",
+// tests/LpmBVETest.cpp:105,236-237); for solid-body rotation the velocity and
+// position errors are added as vertex_/face_velocity_error and
+// vertex_/face_position_error (:67-70, :180-217).
 //
 // Solid-body rotation (default) has an exact solution: velocity Omega z x x
 // and rigid rotation of the Lagrangian coordinates; the driver prints the
@@ -15,6 +22,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iomanip>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -76,7 +85,7 @@ Real rh_phase(const std::vector<Real>& x, const std::vector<Real>& zeta, const s
 int main(int argc, char* argv[]) {
   int depth = 3, output_interval = 0;
   Real dt = 0.01, tfinal = 1.0;
-  std::string ic_name = "sbr";
+  std::string ic_name = "sbr", case_name;
   int scheme = LPM_SCHEME_REF, prec = LPM_PREC_FP64;
   for (int i = 1; i < argc; ++i) {
     const std::string t = argv[i];
@@ -85,6 +94,7 @@ int main(int argc, char* argv[]) {
     else if (t == "-tf" && i + 1 < argc) tfinal = std::stod(argv[++i]);
     else if (t == "-f" && i + 1 < argc) output_interval = std::stoi(argv[++i]);
     else if (t == "-ic" && i + 1 < argc) ic_name = argv[++i];
+    else if (t == "-o" && i + 1 < argc) case_name = argv[++i];
     else if (t == "-scheme" && i + 1 < argc) scheme = std::string(argv[++i]) == "lagr" ? LPM_SCHEME_LAGR : LPM_SCHEME_REF;
     else if (t == "-prec" && i + 1 < argc) prec = std::string(argv[++i]) == "fp32c" ? LPM_PREC_FP32C : LPM_PREC_FP64;
   }
@@ -139,6 +149,38 @@ int main(int argc, char* argv[]) {
     } else {
       phase0 = rh_phase(sphere.face_x, sphere.rel_vort_faces, sphere.face_mask, 4, 0.6);
     }
+    const std::vector<Real> lag_verts = sphere.vert_lag;
+    // XML VTK output of the current (host-mirrored) state
+    auto write_output = [&](int step) {
+      if (case_name.empty()) return;
+      std::vector<BVESphere<IcosTriSphereSeed>::VtkArray> extra;
+      if (!rh) {
+        const Real c = std::cos(Omega * sphere.t), s = std::sin(Omega * sphere.t);
+        auto errors = [&](const std::vector<Real>& x, const std::vector<Real>& u, const std::vector<Real>& lag,
+                          bool faces, const char* where) {
+          const size_t n = x.size() / 3;
+          BVESphere<IcosTriSphereSeed>::VtkArray ve{std::string(where) + "_velocity_error", 3, faces, std::vector<Real>(3 * n)};
+          BVESphere<IcosTriSphereSeed>::VtkArray pe{std::string(where) + "_position_error", 3, faces, std::vector<Real>(3 * n)};
+          for (size_t i = 0; i < n; ++i) {
+            const Real* a = &lag[3 * i];
+            const Real ex[3] = {a[0] * c - a[1] * s, a[1] * c + a[0] * s, a[2]};
+            ve.data[3 * i] = u[3 * i] + Omega * x[3 * i + 1];
+            ve.data[3 * i + 1] = u[3 * i + 1] - Omega * x[3 * i];
+            ve.data[3 * i + 2] = u[3 * i + 2];
+            for (int k = 0; k < 3; ++k) pe.data[3 * i + k] = x[3 * i + k] - ex[k];
+          }
+          extra.push_back(std::move(ve));
+          extra.push_back(std::move(pe));
+        };
+        errors(sphere.vert_x, sphere.vel_verts, lag_verts, false, "vertex");
+        errors(sphere.face_x, sphere.vel_faces, lag_faces, true, "face");
+      }
+      std::ostringstream ss;
+      ss << case_name << IcosTriSphereSeed::faceStr() << depth << "_dt" << dt << "_" << std::setfill('0')
+         << std::setw(4) << step << ".vtp";
+      sphere.outputVtk(ss.str(), extra);
+    };
+    write_output(0);
     int steps_done = 0;
     const int chunk = output_interval > 0 ? output_interval : ntimesteps;
     while (steps_done < ntimesteps) {
@@ -150,6 +192,7 @@ int main(int argc, char* argv[]) {
       if (output_interval > 0) {
         sphere.updateHost();
         std::printf("\tstep %d, t = %g, circulation = %.3e\n", steps_done, sphere.t, sphere.circulation());
+        write_output(steps_done);
       }
     }
     sphere.solve_streamfn(prec, scheme);

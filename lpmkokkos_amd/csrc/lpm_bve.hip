@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -15,6 +16,7 @@
 #include "lpm_bve.h"
 #include "lpm_bve_kernels.hpp"
 #include "lpm_mesh.hpp"
+#include "lpm_meshio.hpp"
 #include "lpm_status.hpp"
 
 namespace lpm {
@@ -1815,20 +1817,27 @@ int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme, hi
   return LPM_OK;
 }
 
-// Before stepping: absolute vorticity for the Lagrangian scheme and a valid
-// entry velocity for (prec, form), queued on `st`.
+// The absolute vorticity zeta + 2 Omega z of the state at the first step with
+// rate `omega` (BVESphere::init_vorticity, src/LpmBVESphere.cpp:133,145):
+// conserved along trajectories, so computed once per rate and kept.  The
+// Lagrangian scheme takes zeta from it; both schemes report it.
+int ctx_ensure_absv(lpm_bve_ctx* c, double omega, hipStream_t st) {
+  if (c->absv_valid && c->absv_omega == omega) return LPM_OK;
+  hipLaunchKernelGGL(k_absv, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, st, c->nt, omega, c->zeta, c->z, c->absv);
+  LPM_HIP(hipGetLastError());
+  c->absv_valid = true;
+  c->absv_omega = omega;
+  return LPM_OK;
+}
+
+// Before stepping: absolute vorticity and a valid entry velocity for
+// (prec, form), queued on `st`.
 int ctx_prepare_step(lpm_bve_ctx* c, double omega, int scheme, int prec, hipStream_t st) {
   const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
-  if (scheme == LPM_SCHEME_LAGR && !(c->absv_valid && c->absv_omega == omega)) {
-    hipLaunchKernelGGL(k_absv, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, st, c->nt, omega, c->zeta, c->z,
-                       c->absv);
-    LPM_HIP(hipGetLastError());
-    c->absv_valid = true;
-    c->absv_omega = omega;
-  }
+  int rc = ctx_ensure_absv(c, omega, st);
+  if (rc) return rc;
   if (!c->vel_valid || (c->vel_prec >= 0 && (c->vel_prec != prec || c->vel_form != form))) {
-    int rc = ctx_eval(c, c->x, c->zeta, prec, form, false, st);
-    if (rc) return rc;
+    if ((rc = ctx_eval(c, c->x, c->zeta, prec, form, false, st))) return rc;
     c->vel_valid = true;
     c->vel_prec = prec;
     c->vel_form = form;
@@ -2188,6 +2197,183 @@ int lpm_bve_get_state(lpm_bve_ctx* c, double* vx, double* vzeta, double* vu, dou
     if (fzeta) fzeta[f] = col(3, i);
     if (fpsi) fpsi[f] = col(7, i);
   }
+  return ok();
+}
+
+}  // extern "C"
+
+namespace lpm {
+namespace {
+
+// The context's state on the host for the file writers: positions,
+// vorticity, velocity and psi (zero when not valid), absolute vorticity for
+// `omega` (computed as the first step at that rate would), areas and mask.
+int ctx_download(lpm_bve_ctx* c, double omega, BveFields& s) {
+  const int64_t nv = c->nv, nf = c->nf;
+  const int nt = c->nt;
+  s = BveFields();
+  s.nv = nv;
+  s.nf = nf;
+  s.vx.resize(3 * nv);
+  s.vzeta.resize(nv);
+  s.vu.resize(3 * nv);
+  s.vpsi.resize(nv);
+  s.fx.resize(3 * nf);
+  s.fzeta.resize(nf);
+  s.fu.resize(3 * nf);
+  s.fpsi.resize(nf);
+  int rc = lpm_bve_get_state(c, s.vx.data(), s.vzeta.data(), s.vu.data(), s.vpsi.data(), s.fx.data(), s.fzeta.data(),
+                             s.fu.data(), s.fpsi.data());
+  if (rc) return rc;
+  if ((rc = ctx_ensure_absv(c, omega, c->stream))) return rc;
+  std::vector<double> absv(nt), area(nt);
+  LPM_HIP(hipMemcpyAsync(absv.data(), c->absv, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream));
+  LPM_HIP(hipMemcpyAsync(area.data(), c->area, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  s.vabsv.assign(absv.begin(), absv.begin() + nv);
+  // divided faces: not evaluated, masked, area 0, absolute vorticity from
+  // their stored vorticity and position
+  s.fabsv.resize(nf);
+  s.farea.assign(nf, 0.0);
+  s.fmask.assign(nf, 1);
+  for (int64_t f = 0; f < nf; ++f) s.fabsv[f] = s.fzeta[f] + 2.0 * omega * s.fx[3 * f + 2];
+  const std::vector<int64_t>& fmap = ctx_fmap(c);
+  for (int j = 0; j < c->ns; ++j) {
+    const int64_t f = fmap[j];
+    s.fmask[f] = 0;
+    s.farea[f] = area[nv + j];
+    s.fabsv[f] = absv[nv + j];
+  }
+  if (!c->psi_valid) {
+    std::fill(s.vpsi.begin(), s.vpsi.end(), 0.0);
+    std::fill(s.fpsi.begin(), s.fpsi.end(), 0.0);
+  }
+  if (!c->vel_valid) {
+    std::fill(s.vu.begin(), s.vu.end(), 0.0);
+    std::fill(s.fu.begin(), s.fu.end(), 0.0);
+  }
+  s.omega = omega;
+  s.vel_prec = c->vel_valid ? c->vel_prec : -2;
+  s.vel_form = c->vel_valid ? c->vel_form : -1;
+  s.psi_valid = c->psi_valid;
+  s.absv_valid = true;
+  s.absv_omega = omega;
+  return LPM_OK;
+}
+
+// The tree the state was built on: `level` >= 0 names it (checked against the
+// state's sizes), -1 takes the cached mesh's level, or none (null) for a
+// state set from arrays.
+int ctx_tree(const lpm_bve_ctx* c, int level, bool required, std::unique_ptr<TreeMesh>& out, const char* who) {
+  out.reset();
+  if (level < 0 && c->mesh) level = c->mesh->level;
+  if (level < 0) {
+    if (required)
+      return fail(LPM_ERR_ARG, std::string(who) + ": face connectivity unknown: pass the tree level the state was built on");
+    return LPM_OK;
+  }
+  if (level > 10) return fail(LPM_ERR_ARG, std::string(who) + ": level must be in [0, 10]");
+  int64_t nvl = IcosTriSphereSeed::nVerticesAtTreeLevel(level), nfl = 0;
+  for (int l = 0; l <= level; ++l) nfl += IcosTriSphereSeed::nFacesAtTreeLevel(l);
+  if (nvl != c->nv || nfl != c->nf)
+    return fail(LPM_ERR_ARG, std::string(who) + ": the state's sizes are not those of tree level " + std::to_string(level));
+  try {
+    out.reset(new TreeMesh(build_icos_tree(level, builtin_icos_seed())));
+  } catch (const std::exception& e) {
+    return fail(LPM_ERR_NOMEM, std::string(who) + ": " + e.what());
+  }
+  return LPM_OK;
+}
+
+// Upload vertex and tree-face values into a target-indexed device array.
+int ctx_upload_targets(lpm_bve_ctx* c, double* dst, int ncomp, const double* vvals, const double* fvals) {
+  const int nt = c->nt;
+  std::vector<double> h((size_t)ncomp * nt);
+  for (int k = 0; k < ncomp; ++k) {
+    for (int64_t i = 0; i < c->nv; ++i) h[k * (size_t)nt + i] = vvals[ncomp * i + k];
+    const std::vector<int64_t>& fmap = ctx_fmap(c);
+    for (int j = 0; j < c->ns; ++j) h[k * (size_t)nt + c->nv + j] = fvals[ncomp * fmap[j] + k];
+  }
+  LPM_HIP(hipMemcpyAsync(dst, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  return LPM_OK;
+}
+
+}  // namespace
+}  // namespace lpm
+
+extern "C" {
+
+int lpm_bve_checkpoint(lpm_bve_ctx* c, const char* path, int level, double time, double omega) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_checkpoint: no state");
+  if (!path) return fail(LPM_ERR_ARG, "lpm_bve_checkpoint: null path");
+  LPM_HIP(hipSetDevice(c->device));
+  std::unique_ptr<TreeMesh> tree;
+  int rc = ctx_tree(c, level, false, tree, "lpm_bve_checkpoint");
+  if (rc) return rc;
+  BveFields s;
+  if ((rc = ctx_download(c, omega, s))) return rc;
+  s.time = time;
+  std::string err;
+  if (!write_bve_nc(path, s, tree.get(), err)) return fail(LPM_ERR_IO, "lpm_bve_checkpoint: " + err);
+  return ok();
+}
+
+int lpm_bve_restore(lpm_bve_ctx* c, const char* path, double* time, double* omega) {
+  if (!c || !path) return fail(LPM_ERR_ARG, "lpm_bve_restore: null argument");
+  BveFields s;
+  std::string err;
+  if (!read_bve_nc(path, s, err)) return fail(LPM_ERR_IO, "lpm_bve_restore: " + err);
+  int rc = lpm_bve_set_state(c, s.nv, s.vx.data(), s.vzeta.data(), s.nf, s.fx.data(), s.fzeta.data(), s.farea.data(),
+                             s.fmask.data());
+  if (rc) return rc;
+  if (s.vel_prec > -2) {
+    if ((rc = ctx_upload_targets(c, c->u, 3, s.vu.data(), s.fu.data()))) return rc;
+    c->vel_valid = true;
+    c->vel_prec = s.vel_prec;
+    c->vel_form = s.vel_form;
+  }
+  if (s.psi_valid) {
+    if ((rc = ctx_upload_targets(c, c->psi, 1, s.vpsi.data(), s.fpsi.data()))) return rc;
+    c->psi_valid = true;
+  }
+  if (s.absv_valid) {
+    if ((rc = ctx_upload_targets(c, c->absv, 1, s.vabsv.data(), s.fabsv.data()))) return rc;
+    c->absv_valid = true;
+    c->absv_omega = s.absv_omega;
+  }
+  if (time) *time = s.time;
+  if (omega) *omega = s.omega;
+  return ok();
+}
+
+int lpm_bve_write_vtk(lpm_bve_ctx* c, const char* path, int level, double omega, int format) {
+  return lpm_bve_write_vtk_arrays(c, path, level, omega, format, nullptr, 0);
+}
+
+int lpm_bve_write_vtk_arrays(lpm_bve_ctx* c, const char* path, int level, double omega, int format,
+                             const lpm_vtk_array* arrays, int narrays) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_write_vtk: no state");
+  if (!path) return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: null path");
+  if (format < LPM_VTK_LEGACY_ASCII || format > LPM_VTK_XML_BINARY)
+    return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: unknown format");
+  if (narrays < 0 || (narrays > 0 && !arrays)) return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: bad array list");
+  std::vector<VtkExtra> extra;
+  for (int k = 0; k < narrays; ++k) {
+    const lpm_vtk_array& a = arrays[k];
+    if (!a.name || !a.name[0] || std::strpbrk(a.name, " \t\n\r\"<>&") || a.ncomp < 1 || a.ncomp > 9 || !a.data)
+      return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: array " + std::to_string(k) +
+                                   ": name must be non-empty without blanks or markup, 1 <= ncomp <= 9, data non-null");
+    extra.push_back({a.name, a.ncomp, a.on_faces != 0, a.data});
+  }
+  LPM_HIP(hipSetDevice(c->device));
+  std::unique_ptr<TreeMesh> tree;
+  int rc = ctx_tree(c, level, true, tree, "lpm_bve_write_vtk");
+  if (rc) return rc;
+  BveFields s;
+  if ((rc = ctx_download(c, omega, s))) return rc;
+  std::string err;
+  if (!write_bve_vtk(path, s, *tree, format, err, extra)) return fail(LPM_ERR_IO, "lpm_bve_write_vtk: " + err);
   return ok();
 }
 

@@ -6,6 +6,7 @@
 #include "lpm_bve.h"
 #include "lpm_ic.hpp"
 #include "lpm_mesh.hpp"
+#include "lpm_meshio.hpp"
 #include "lpm_status.hpp"
 
 namespace lpm {
@@ -71,6 +72,76 @@ int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vla
     }
   } catch (const std::exception& e) {
     return fail(LPM_ERR_NOMEM, std::string("lpm_icos_tree_init: ") + e.what());
+  }
+  return ok();
+}
+
+int lpm_mesh_write_nc(int level, const char* seed_path, const char* path) {
+  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_mesh_write_nc: level must be in [0, 12]");
+  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_write_nc: null path");
+  try {
+    SeedTables file_seed;
+    const SeedTables* seed = &builtin_icos_seed();
+    if (seed_path) {
+      std::string err;
+      if (!read_icos_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_nc: " + err);
+      seed = &file_seed;
+    }
+    const TreeMesh m = build_icos_tree(level, *seed);
+    nc::Writer w;
+    add_mesh_vars(w, m, nullptr, nullptr);
+    std::string err;
+    if (!w.write(path, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_nc: " + err);
+  } catch (const std::exception& e) {
+    return fail(LPM_ERR_NOMEM, std::string("lpm_mesh_write_nc: ") + e.what());
+  }
+  return ok();
+}
+
+int lpm_mesh_read_nc(const char* path, double* vx, double* vlag, double* fx, double* flag, double* farea,
+                     uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts) {
+  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_read_nc: null path");
+  nc::Reader r;
+  std::string err;
+  if (!r.open(path, err)) return fail(LPM_ERR_IO, "lpm_mesh_read_nc: " + err);
+  const int64_t nv = r.dim("nverts"), ne = r.dim("nedges"), nf = r.dim("nfaces");
+  if (nv < 0 || nf < 1 || r.dim("ndim") != 3)
+    return fail(LPM_ERR_IO, "lpm_mesh_read_nc: missing or bad nverts / nfaces / ndim dimensions");
+  std::vector<uint8_t> mask;
+  if (!r.read("face_mask", mask) || (int64_t)mask.size() != nf)
+    return fail(LPM_ERR_IO, "lpm_mesh_read_nc: missing or mis-sized variable face_mask");
+  auto get_d = [&](const char* name, double* out, int64_t n) -> bool {
+    if (!out) return true;
+    std::vector<double> v;
+    if (!r.read(name, v) || (int64_t)v.size() != n) {
+      err = name;
+      return false;
+    }
+    std::memcpy(out, v.data(), sizeof(double) * n);
+    return true;
+  };
+  auto get_i = [&](const char* name, int32_t* out, int64_t n) -> bool {
+    if (!out) return true;
+    std::vector<int32_t> v;
+    if (!r.read(name, v) || (int64_t)v.size() != n) {
+      err = name;
+      return false;
+    }
+    std::memcpy(out, v.data(), sizeof(int32_t) * n);
+    return true;
+  };
+  if (!get_d("phys_crds_verts", vx, 3 * nv) || !get_d("lag_crds_verts", vlag, 3 * nv) ||
+      !get_d("phys_crds_faces", fx, 3 * nf) || !get_d("lag_crds_faces", flag, 3 * nf) ||
+      !get_d("face_area", farea, nf) || !get_i("face_verts", fverts, 3 * nf) || !get_i("face_parents", fparent, nf))
+    return fail(LPM_ERR_IO, "lpm_mesh_read_nc: missing or mis-sized variable " + err);
+  if (fmask) std::memcpy(fmask, mask.data(), nf);
+  if (counts) {
+    int64_t leaves = 0;
+    for (uint8_t b : mask) leaves += b == 0;
+    counts[0] = nv;
+    counts[1] = ne;
+    counts[2] = nf;
+    counts[3] = leaves;
   }
   return ok();
 }

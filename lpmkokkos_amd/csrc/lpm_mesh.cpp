@@ -264,6 +264,31 @@ TreeMesh build_icos_tree(int lev, const SeedTables& seed) {
       if (!(b.f_kid0[f] > 0)) b.split_face(f);
   }
   m.n_edges = (int64_t)b.e_orig.size();
+  // topology: edges with parents derived from their kids; faces with kids
+  // (consecutive, first at f_kid0) and tree levels
+  const int64_t ne_all = m.n_edges, nf_all = m.nfaces();
+  m.edge_orig = b.e_orig;
+  m.edge_dest = b.e_dest;
+  m.edge_left = b.e_left;
+  m.edge_right = b.e_right;
+  m.edge_parent.assign(ne_all, kNone);
+  m.edge_kids.assign(2 * ne_all, kNone);
+  for (int64_t e = 0; e < ne_all; ++e) {
+    m.edge_kids[2 * e] = b.e_kid0[e];
+    m.edge_kids[2 * e + 1] = b.e_kid1[e];
+    if (b.e_kid0[e] != kNone) {
+      m.edge_parent[b.e_kid0[e]] = (int)e;
+      m.edge_parent[b.e_kid1[e]] = (int)e;
+    }
+  }
+  m.face_edges = b.f_edges;
+  m.face_kids.assign(4 * nf_all, kNone);
+  m.face_level.assign(nf_all, 0);
+  for (int64_t f = 0; f < nf_all; ++f) {
+    if (m.face_parent[f] != kNone) m.face_level[f] = m.face_level[m.face_parent[f]] + 1;
+    if (b.f_kid0[f] != kNone)
+      for (int k = 0; k < 4; ++k) m.face_kids[4 * f + k] = b.f_kid0[f] + k;
+  }
   return m;
 }
 

@@ -62,6 +62,13 @@ struct TreeMesh {
   std::vector<int> face_verts;   // [nfaces][3]
   std::vector<int> face_parent;  // -1 for seed faces
   int64_t n_edges = 0;
+  // topology of Edges and Faces (src/LpmEdges.hpp, src/LpmFaces.hpp); -1 is
+  // the reference's NULL_IND (src/LpmDefs.hpp:102)
+  std::vector<int> edge_orig, edge_dest, edge_left, edge_right, edge_parent;
+  std::vector<int> edge_kids;   // [nedges][2]
+  std::vector<int> face_edges;  // [nfaces][3]
+  std::vector<int> face_kids;   // [nfaces][4]
+  std::vector<int> face_level;  // tree level, 0 for seed faces
   int64_t nverts() const { return (int64_t)vert_phys.size(); }
   int64_t nfaces() const { return (int64_t)face_phys.size(); }
   int64_t nleaves() const;

@@ -17,6 +17,13 @@ the decomposition it has alone and writes only its own partial slots.
 Requests below level 8 share the batched lane; level-8 requests run in lanes
 of their own (see Server), so short requests are not held to a level-8 step.
 
+Checkpoint and resume: run(..., stop_after_steps=k, checkpoint_dir=d) stops
+admitting after k batched steps, lets the large lanes finish, writes every
+batched request's state at its step boundary to a netCDF checkpoint
+(Context.checkpoint) and the queue to d/manifest.json; load_manifest(d)
+returns the requests for a new server, whose results are those the first
+server would have produced, bit for bit.
+
 Reported: completion latency (finish - arrival) p50 / p99 and aggregate
 pair interactions per second over the run.
 
@@ -25,8 +32,10 @@ pair interactions per second over the run.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
+import os
 import sys
 import time
 from dataclasses import dataclass, field
@@ -56,6 +65,7 @@ class Request:
     admit_s: float = 0.0  # wall time of building the request's state
     pairs: float = 0.0
     result: dict | None = field(default=None, repr=False)
+    checkpoint: str | None = None  # resume from this file (steps_done steps already taken)
 
     @property
     def latency(self) -> float:
@@ -90,6 +100,21 @@ def make_requests(n: int, rate: float, seed: int = 4, level_probs=None, steps: i
                           nvort=4 + int(13 * u[5 * i + 4]), amp=0.05, sigma=0.15, vortex_seed=100000 * seed + i)
         reqs.append(Request(rid=i, arrival=t, spec=spec, steps=steps))
     return reqs
+
+
+def load_manifest(directory: str) -> list[Request]:
+    """The requests a stopped server left (run(..., checkpoint_dir=directory)):
+    started ones resume from their checkpoints and arrive at once; the others
+    keep their arrival offsets past the stop."""
+    with open(os.path.join(directory, "manifest.json")) as f:
+        doc = json.load(f)
+    out = []
+    for e in doc["requests"]:
+        ck = e["checkpoint"]
+        out.append(Request(rid=e["rid"], arrival=e["arrival"], spec=MemberSpec(**e["spec"]), steps=e["steps"],
+                           steps_done=e["steps_done"],
+                           checkpoint=os.path.join(directory, ck) if ck else None))
+    return out
 
 
 def request_bytes(spec: MemberSpec) -> int:
@@ -143,13 +168,18 @@ class Server:
         t = time.perf_counter()
         # batched-lane members queue on the ensemble's stream; a large request
         # gets a stream of its own
-        r.ctx = make_member(r.spec, self.device, stream=None if large else self.ens_stream)
+        stream = None if large else self.ens_stream
+        if r.checkpoint:
+            r.ctx = Context(self.device, stream=stream)
+            r.ctx.restore(r.checkpoint)
+        else:
+            r.ctx = make_member(r.spec, self.device, stream=stream)
         r.admit_s = time.perf_counter() - t
         r.bytes = request_bytes(r.spec)
         self.used += r.bytes
         r.admitted = now
         if large:
-            r.ctx.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=r.steps)
+            r.ctx.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=r.steps - r.steps_done)
             self.large.append(r)
         else:
             self.ens.add(r.ctx)
@@ -177,7 +207,39 @@ class Server:
         self.retired.clear()
         self.retired_bytes = 0
 
-    def run(self, requests: list[Request]) -> dict:
+    def _suspend(self, pending: list[Request], now: float, directory: str):
+        """Checkpoint every batched request at its step boundary, release it,
+        and write the queue (with the pending requests) to the manifest."""
+        os.makedirs(directory, exist_ok=True)
+        entries = []
+        for r in list(self.batch):
+            name = f"request_{r.rid}.nc"
+            r.ctx.checkpoint(os.path.join(directory, name), time=r.steps_done * self.dt, omega=self.omega,
+                             level=r.spec.level)
+            self.ens.remove(r.ctx)
+            self.batch.remove(r)
+            self.retired.append(r.ctx)
+            r.ctx = None
+            self.used -= r.bytes
+            self.retired_bytes += r.bytes
+            entries.append((r, name, 0.0))
+        entries += [(r, None, max(r.arrival - now, 0.0)) for r in pending]
+        doc = {"dt": self.dt, "omega": self.omega, "prec": self.prec,
+               "requests": [{"rid": r.rid, "arrival": arr, "spec": dataclasses.asdict(r.spec), "steps": r.steps,
+                             "steps_done": r.steps_done if name else 0, "checkpoint": name}
+                            for r, name, arr in entries]}
+        with open(os.path.join(directory, "manifest.json"), "w") as f:
+            json.dump(doc, f, indent=1)
+        return len(entries)
+
+    def run(self, requests: list[Request], stop_after_steps: int | None = None,
+            checkpoint_dir: str | None = None) -> dict:
+        """Serve `requests` to completion, or, with stop_after_steps, stop after
+        that many batched steps: admit no more, let the large lanes finish, and
+        checkpoint what is left into checkpoint_dir (see load_manifest)."""
+        if stop_after_steps is not None and not checkpoint_dir:
+            raise ValueError("stop_after_steps needs a checkpoint_dir")
+        suspended = 0
         for r in requests:
             if request_bytes(r.spec) > self.budget:
                 raise ValueError(f"request {r.rid} (level {r.spec.level}) exceeds the memory budget")
@@ -217,6 +279,14 @@ class Server:
                     done.append(r)
             if self.retired and not batch_inflight and not self.large:
                 self._free_retired()
+            if stop_after_steps is not None and steps >= stop_after_steps and not batch_inflight:
+                if not self.large:
+                    suspended = self._suspend(pending, now, checkpoint_dir)
+                    pending = []
+                    self._free_retired()
+                    break
+                time.sleep(0.0005)
+                continue
             # admission in arrival order while a slot and memory allow; the
             # batched lane changes membership only between steps
             while pending and pending[0].arrival <= now and len(self.active) < self.max_active:
@@ -243,9 +313,10 @@ class Server:
                 time.sleep(0.0005)
         wall = time.perf_counter() - t0
         self._free_retired()
-        lat = np.array([r.latency for r in done])
+        lat = np.array([r.latency for r in done] or [0.0])
         return {
             "requests": len(done),
+            "suspended": suspended,
             "wall_s": wall,
             "device_busy_s": busy,
             "batched_steps": steps,
@@ -258,8 +329,8 @@ class Server:
             "latency_mean_s": float(lat.mean()),
             "latency_p50_s_below_large": float(np.percentile([r.latency for r in done if r.spec.level < self.large_level]
                                                              or [0.0], 50)),
-            "admission_mean_s": float(np.mean([r.admit_s for r in done])),
-            "admission_max_s": float(np.max([r.admit_s for r in done])),
+            "admission_mean_s": float(np.mean([r.admit_s for r in done] or [0.0])),
+            "admission_max_s": float(np.max([r.admit_s for r in done] or [0.0])),
             "levels": {int(l): int(sum(1 for r in done if r.spec.level == l)) for l in sorted({r.spec.level for r in done})},
         }
 

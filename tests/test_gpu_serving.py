@@ -57,3 +57,30 @@ def test_large_lane_does_not_hold_short_requests():
     print(f"large request {big.latency:.2f} s, slowest short request {worst_small:.3f} s")
     assert big.finished > max(r.finished for r in small)  # the short ones overtook it
     assert worst_small < 0.25 * big.latency
+
+
+def test_suspend_and_resume_match_alone(tmp_path):
+    """A server stopped after a few batched steps checkpoints its batched
+    requests and queue; a second server resumes them.  Every request's result,
+    whichever server finished it, equals the request run alone, bit for bit."""
+    reqs = S.make_requests(8, rate=400.0, seed=21, level_probs={2: 0.4, 3: 0.4, 4: 0.2}, steps=6)
+    a = S.Server(keep_results=True, large_level=4)
+    res_a = a.run(reqs, stop_after_steps=3, checkpoint_dir=str(tmp_path))
+    a.close()
+    finished = {r.rid: r for r in reqs if r.result is not None}
+    assert res_a["requests"] == len(finished) and res_a["suspended"] == 8 - len(finished) > 0
+    resumed = S.load_manifest(str(tmp_path))
+    assert any(r.checkpoint and 0 < r.steps_done < r.steps for r in resumed)
+    b = S.Server(keep_results=True, large_level=4)
+    res_b = b.run(resumed)
+    b.close()
+    assert res_b["requests"] == len(resumed) and res_b["suspended"] == 0
+    finished.update({r.rid: r for r in resumed})
+    assert sorted(finished) == list(range(8))
+    for rid, r in finished.items():
+        c = E.make_member(r.spec)
+        c.advance(0.005, 2 * math.pi, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=r.steps)
+        s = c.get_state()
+        for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
+            assert np.array_equal(s[k], r.result[k]), (rid, k)
+        c.close()
