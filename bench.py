@@ -39,7 +39,7 @@ FLOPS_PER_PAIR = 15
 FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # HBM bytes per level-8 FP32C pair launch: FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc
 # runs, each scaled by the same counter's reading for a device copy of known size
-TRAFFIC_BYTES_PER_LAUNCH = 10.06e9 + 126.1e6
+TRAFFIC_BYTES_PER_LAUNCH = 11.72e9 + 126.1e6
 DT = 0.005
 OMEGA = 2 * np.pi
 

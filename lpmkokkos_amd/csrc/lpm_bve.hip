@@ -79,10 +79,18 @@ __global__ __launch_bounds__(kBlock) void k_prep_f64_dot(Src64* __restrict__ src
   src[j] = r;
 }
 
-// FP32C source record: a = {dx, dy, dz, |d|^2} with d = fl32(y - c) the
-// offset from the tile centre c; b = {q, q dx, q dy, q dz} with
-// q = fl32((-zeta A)/(2 pi)).  |d|^2 and q d are formed in FP32 from the
-// stored offsets so that both pair forms see the same source position.
+// FP32C source record: a = {dx, dy, dz, q y.n}, b = {q y.e1, q y.e2, q, |d|^2}
+// with d = fl32(y - c) the offset from the tile centre c,
+// q = fl32((-zeta A)/(2 pi)) and y = c + d written in the tile's orthonormal
+// frame (n along c, e1, e2; FP64, rounded once).  The far forms sum
+// {a.w, b.x, b.y}/|d|^2, i.e. q y/|d|^2 in frame components (the velocity
+// form reads a and b.xy: one 16-byte and one 8-byte LDS read), and the tile's frame
+// maps the three sums back in FP64: the component along x that this leaves
+// out of q (y - x)/|d|^2 is cancelled by the cross product with x, and the
+// large coefficient (along n) is rounded relative to itself, so its error
+// stays in proportion to the term's own size after the cross product.
+// |d|^2 is formed in FP32 from the stored offsets so that both pair forms
+// see the same source position.
 struct Src32 {
   float4 a, b;
 };
@@ -92,9 +100,37 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Orthonormal frame of a tile: n = c/|c| (e_z when |c| is tiny), e1 = the
+// normalised n x (the axis of n's smallest component), e2 = n x e1.
+__device__ __forceinline__ void tile_frame(const double (&c)[4], double (&n)[3], double (&e1)[3], double (&e2)[3]) {
+  const double cn = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  if (cn > 1e-3) {
+    for (int k = 0; k < 3; ++k) n[k] = c[k] / cn;
+  } else {
+    n[0] = 0.0;
+    n[1] = 0.0;
+    n[2] = 1.0;
+  }
+  const double a0 = fabs(n[0]), a1 = fabs(n[1]), a2 = fabs(n[2]);
+  double v[3];
+  if (a0 <= a1 && a0 <= a2) {
+    v[0] = 0.0, v[1] = n[2], v[2] = -n[1];
+  } else if (a1 <= a2) {
+    v[0] = -n[2], v[1] = 0.0, v[2] = n[0];
+  } else {
+    v[0] = n[1], v[1] = -n[0], v[2] = 0.0;
+  }
+  const double vn = 1.0 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int k = 0; k < 3; ++k) e1[k] = v[k] * vn;
+  e2[0] = n[1] * e1[2] - n[2] * e1[1];
+  e2[1] = n[2] * e1[0] - n[0] * e1[2];
+  e2[2] = n[0] * e1[1] - n[1] * e1[0];
+}
+
 // One workgroup per tile: tile centre (FP64 mean of its sources), the
-// tile's FP64 sum of q, its radius max |d| (rounded up), and the records.
-// Padding records sit 8 units from the centre with q = 0.
+// tile's FP64 sum of q, its radius max |d| (rounded up), its frame (at
+// ctr[ntiles + 3t + {0,1,2}] = n, e1, e2), and the records.  Padding records
+// sit at the origin with q = 0.
 __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, double4* __restrict__ ctr,
                                                      float* __restrict__ trad, double* __restrict__ srcq, Pos3 pos,
                                                      const double* __restrict__ zeta,
@@ -128,18 +164,27 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, do
   c[0] *= inv;
   c[1] *= inv;
   c[2] *= inv;
+  // every lane forms the same frame from the same sums
+  double n[3], e1[3], e2[3];
+  tile_frame(c, n, e1, e2);
   Src32 r;
   double rad = 0.0;
   if (valid) {
     const float dx = (float)(p0 - c[0]), dy = (float)(p1 - c[1]), dz = (float)(p2 - c[2]);
     const float qf = (float)q;
     const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    r.a = make_float4(dx, dy, dz, d2);
-    r.b = make_float4(qf, qf * dx, qf * dy, qf * dz);
+    // y = c + d as the kernels see it, in frame components
+    const double y0 = c[0] + (double)dx, y1 = c[1] + (double)dy, y2 = c[2] + (double)dz, qd = qf;
+    r.a = make_float4(dx, dy, dz, (float)(qd * (y0 * n[0] + y1 * n[1] + y2 * n[2])));
+    r.b = make_float4((float)(qd * (y0 * e1[0] + y1 * e1[1] + y2 * e1[2])),
+                      (float)(qd * (y0 * e2[0] + y1 * e2[1] + y2 * e2[2])), qf, d2);
     rad = sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
   } else {
-    r.a = make_float4(8.0f, 8.0f, 8.0f, 192.0f);
-    r.b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // at the origin (offset -c): 1 - x.y = 1 for every target x, so no
+    // padding denominator can vanish and spoil a shared reciprocal
+    const float dx = (float)(-c[0]), dy = (float)(-c[1]), dz = (float)(-c[2]);
+    r.a = make_float4(dx, dy, dz, 0.0f);
+    r.b = make_float4(0.0f, 0.0f, 0.0f, fmaf(dx, dx, fmaf(dy, dy, dz * dz)));
   }
   src[j] = r;
   srcq[j] = q;
@@ -149,6 +194,10 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, do
   __syncthreads();
   if (threadIdx.x == 0) {
     ctr[t] = make_double4(c[0], c[1], c[2], c[3]);
+    double4* fr = ctr + gridDim.x + 3 * (long)t;
+    fr[0] = make_double4(n[0], n[1], n[2], 0.0);
+    fr[1] = make_double4(e1[0], e1[1], e1[2], 0.0);
+    fr[2] = make_double4(e2[0], e2[1], e2[2], 0.0);
     const double rr = fmax(fmax(red[4][0], red[4][1]), fmax(red[4][2], red[4][3]));
     trad[t] = (float)(rr * (1.0 + 1e-6));
   }
@@ -414,7 +463,7 @@ template <bool PSI, bool CHECK>
 __device__ __forceinline__ void pair32_near(const float (&D)[3], const Src32& s, bool me, float (&T)[kNC]) {
   const float e0 = D[0] + s.a.x, e1 = D[1] + s.a.y, e2 = D[2] + s.a.z;
   float r2 = fmaf(e0, e0, fmaf(e1, e1, e2 * e2));
-  float q = s.b.x;
+  float q = s.b.z;
   if (CHECK && me) {
     r2 = 1.0f;
     q = 0.0f;
@@ -426,21 +475,29 @@ __device__ __forceinline__ void pair32_near(const float (&D)[3], const Src32& s,
   if (PSI) T[3] = fmaf(q, __builtin_amdgcn_logf(r2), T[3]);
 }
 
-// Far form: |d|^2 = |D|^2 + |dx|^2 + 2 D.dx from the precomputed |dx|^2;
-// accumulates F = sum q/|d|^2 and G = sum q dx/|d|^2, so that the tile's
-// sum of q d/|d|^2 is D F + G.  8 VALU operations per pair.
-template <bool PSI>
-__device__ __forceinline__ void pair32_far(const float (&D2)[3], float E, const Src32& s, float (&F)[kNC + 1]) {
-  float r2 = E + s.a.w;
+// Far form with psi: |d|^2 = |D|^2 + |dx|^2 + 2 D.dx from the precomputed
+// |dx|^2; F[0..2] accumulate q y/|d|^2 in frame components (see Src32),
+// F[3] accumulates q log2 |d|^2.
+__device__ __forceinline__ void pair32_far_psi(const float (&D2)[3], float E, const Src32& s, float (&F)[kNC]) {
+  float r2 = E + s.b.w;
   r2 = fmaf(D2[0], s.a.x, r2);
   r2 = fmaf(D2[1], s.a.y, r2);
   r2 = fmaf(D2[2], s.a.z, r2);
   const float inv = __builtin_amdgcn_rcpf(r2);
-  F[0] = fmaf(inv, s.b.x, F[0]);
-  F[1] = fmaf(inv, s.b.y, F[1]);
-  F[2] = fmaf(inv, s.b.z, F[2]);
-  F[3] = fmaf(inv, s.b.w, F[3]);
-  if (PSI) F[4] = fmaf(s.b.x, __builtin_amdgcn_logf(r2), F[4]);
+  F[0] = fmaf(inv, s.a.w, F[0]);
+  F[1] = fmaf(inv, s.b.x, F[1]);
+  F[2] = fmaf(inv, s.b.y, F[2]);
+  F[3] = fmaf(s.b.z, __builtin_amdgcn_logf(r2), F[3]);
+}
+
+// S += scale (n F0 + e1 F1 + e2 F2) in FP64 with the tile frame fr[0..2].
+__device__ __forceinline__ void add_frame(double (&S)[kNC], const double4* fr, float F0, float F1, float F2,
+                                          double scale) {
+  const double4 n = fr[0], e1 = fr[1], e2 = fr[2];
+  const double p0 = scale * F0, p1 = scale * F1, p2 = scale * F2;
+  S[0] += fma(n.x, p0, fma(e1.x, p1, e2.x * p2));
+  S[1] += fma(n.y, p0, fma(e1.y, p1, e2.y * p2));
+  S[2] += fma(n.z, p0, fma(e1.z, p1, e2.z * p2));
 }
 
 // Far form of the step kernel (velocity only).  On the unit sphere
@@ -449,30 +506,30 @@ __device__ __forceinline__ void pair32_far(const float (&D2)[3], float E, const 
 // where A = 1 - x.c is formed per target and tile in FP64 (no cancellation
 // there) and rounded once, and x.dx is small: on a tile beyond 4 tile radii
 // the cancellation in den is at most ~4, as for |D|^2 + |dx|^2 + 2 D.dx.
-// 3 FMAs per pair; F = sum q/den and G = sum q dx/den give the tile's
-// sum of q d/|d|^2 as (D F + G) / 2.
+// 3 FMAs per pair; the three sums of b.{y,z,w}/den, halved and mapped by
+// the tile frame, give the tile's sum of q d/|d|^2 up to a component along
+// x (see Src32).
 __device__ __forceinline__ float far_den(const float (&NX)[3], float A, const Src32& s) {
   float v = fmaf(NX[0], s.a.x, A);
   v = fmaf(NX[1], s.a.y, v);
   return fmaf(NX[2], s.a.z, v);
 }
 
-__device__ __forceinline__ void far_acc(float inv, const Src32& s, float (&F)[kNC + 1]) {
-  F[0] = fmaf(inv, s.b.x, F[0]);
-  F[1] = fmaf(inv, s.b.y, F[1]);
-  F[2] = fmaf(inv, s.b.z, F[2]);
-  F[3] = fmaf(inv, s.b.w, F[3]);
+__device__ __forceinline__ void far_acc(float inv, const Src32& s, float (&F)[3]) {
+  F[0] = fmaf(inv, s.a.w, F[0]);
+  F[1] = fmaf(inv, s.b.x, F[1]);
+  F[2] = fmaf(inv, s.b.y, F[2]);
 }
 
 // Four pairs sharing one reciprocal: with a, b, c, d the four denominators,
 // p = 1/(ab cd), 1/a = b (cd p), 1/b = a (cd p), 1/c = d (ab p),
 // 1/d = c (ab p).  9 multiplies and one v_rcp_f32 replace four v_rcp_f32,
 // which issue at a quarter of the FMA rate and do not overlap it.  Far-form
-// denominators lie in [((near_factor - 1) r_tile)^2 / 2, 126] (126 for
-// padding records), so the product of four stays in the FP32 normal range;
-// each reciprocal carries about 5 roundings instead of 1.
+// denominators lie in [((near_factor - 1) r_tile)^2 / 2, 2] (1 for padding
+// records), so the product of four stays in the FP32 normal range; each
+// reciprocal carries about 5 roundings instead of 1.
 __device__ __forceinline__ void pair32_far4(const float (&NX)[3], float A, const Src32& s0, const Src32& s1,
-                                            const Src32& s2, const Src32& s3, float (&F)[kNC + 1]) {
+                                            const Src32& s2, const Src32& s3, float (&F)[3]) {
   const float a = far_den(NX, A, s0), b = far_den(NX, A, s1);
   const float c = far_den(NX, A, s2), d = far_den(NX, A, s3);
   const float ab = a * b, cd = c * d;
@@ -486,8 +543,8 @@ __device__ __forceinline__ void pair32_far4(const float (&NX)[3], float A, const
 
 // The same for two targets (A, B) and two sources.
 __device__ __forceinline__ void pair32_far22(const float (&NXA)[3], float AA, const float (&NXB)[3], float AB,
-                                             const Src32& s0, const Src32& s1, float (&FA)[kNC + 1],
-                                             float (&FB)[kNC + 1]) {
+                                             const Src32& s0, const Src32& s1, float (&FA)[3],
+                                             float (&FB)[3]) {
   const float a = far_den(NXA, AA, s0), b = far_den(NXA, AA, s1);
   const float c = far_den(NXB, AB, s0), d = far_den(NXB, AB, s1);
   const float ab = a * b, cd = c * d;
@@ -499,20 +556,14 @@ __device__ __forceinline__ void pair32_far22(const float (&NXA)[3], float AA, co
   far_acc(c * icd, s1, FB);
 }
 
-// The far form reads 28 of a record's 32 bytes; keeping the unused |dx|^2
-// live makes each half a single 16-byte LDS read instead of 12 + 16 bytes.
-__device__ __forceinline__ Src32 load_src(const Src32* p) {
-  Src32 r = *p;
-  asm volatile("" ::"v"(r.a.w));
-  return r;
-}
+__device__ __forceinline__ Src32 load_src(const Src32* p) { return *p; }
 
 // Far-form sums of one tile for R targets (velocity only), one reciprocal per
 // four pairs: target pairs x source pairs, and four sources for the last
 // target when R is odd.
 template <int R>
 __device__ __forceinline__ void far_tile(const Src32* ts, const float (&NX)[R][3], const float (&A)[R],
-                                         float (&F)[R][kNC + 1]) {
+                                         float (&F)[R][3]) {
   if constexpr (R == 1) {
     for (int j = 0; j < kTile; j += 4) {
       const Src32 s0 = load_src(ts + j), s1 = load_src(ts + j + 1), s2 = load_src(ts + j + 2),
@@ -520,6 +571,7 @@ __device__ __forceinline__ void far_tile(const Src32* ts, const float (&NX)[R][3
       pair32_far4(NX[0], A[0], s0, s1, s2, s3, F[0]);
     }
   } else if constexpr (R % 2 == 0) {
+#pragma unroll 2
     for (int j = 0; j < kTile; j += 2) {
       const Src32 s0 = load_src(ts + j), s1 = load_src(ts + j + 1);
 #pragma unroll
@@ -539,25 +591,42 @@ __device__ __forceinline__ void far_tile(const Src32* ts, const float (&NX)[R][3
   }
 }
 
-// A tile's far-form contribution T for R targets (velocity-only kernels):
-// A = 1 - x.c in FP64, the sums, then (D F + G) / 2.
+// A tile's far-form contribution for R targets (velocity-only kernels),
+// added to S: A = 1 - x.c in FP64, the sums, then the frame in FP64.
 template <int R>
 __device__ __forceinline__ void far_tile_sum(const Src32* ts, const double (&x)[R][3], const double4& c,
-                                             const float (&NX)[R][3], const float (&D)[R][3],
-                                             float (&T)[R][kNC]) {
-  float A[R], F[R][kNC + 1];
+                                             const double4* fr, const float (&NX)[R][3], double (&S)[R][kNC]) {
+  float A[R], F[R][3];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     A[r] = (float)(1.0 - (x[r][0] * c.x + x[r][1] * c.y + x[r][2] * c.z));
-    for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0f;
+    for (int k = 0; k < 3; ++k) F[r][k] = 0.0f;
   }
   far_tile<R>(ts, NX, A, F);
 #pragma unroll
+  for (int r = 0; r < R; ++r) add_frame(S[r], fr, F[r][0], F[r][1], F[r][2], 0.5);
+}
+
+// A tile's far-form contribution with psi for R targets, added to S.
+template <int R>
+__device__ __forceinline__ void far_tile_psi(const Src32* ts, const float (&D)[R][3], const float (&E)[R],
+                                             const double4* fr, double (&S)[R][kNC]) {
+  float D2[R][3], F[R][kNC];
+#pragma unroll
   for (int r = 0; r < R; ++r) {
-    T[r][0] = 0.5f * fmaf(D[r][0], F[r][0], F[r][1]);
-    T[r][1] = 0.5f * fmaf(D[r][1], F[r][0], F[r][2]);
-    T[r][2] = 0.5f * fmaf(D[r][2], F[r][0], F[r][3]);
-    T[r][3] = 0.0f;
+    for (int k = 0; k < 3; ++k) D2[r][k] = 2.0f * D[r][k];
+    for (int k = 0; k < kNC; ++k) F[r][k] = 0.0f;
+  }
+#pragma unroll 4
+  for (int j = 0; j < kTile; ++j) {
+    const Src32 sj = ts[j];
+#pragma unroll
+    for (int r = 0; r < R; ++r) pair32_far_psi(D2[r], E[r], sj, F[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    add_frame(S[r], fr, F[r][0], F[r][1], F[r][2], 1.0);
+    S[r][3] += (double)F[r][3];
   }
 }
 
@@ -599,9 +668,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
       near |= E[r] < rn * rn;
     }
     const Src32* ts = src + base;
-    float T[R][kNC];
+    const double4* fr = ctr + a.ntiles + 3 * (long)t;
     const bool has_self = wave_has_self<R>(self, base);
     if (has_self || __ballot(near) != 0) {
+      float T[R][kNC];
 #pragma unroll
       for (int r = 0; r < R; ++r)
         for (int k = 0; k < kNC; ++k) T[r][k] = 0.0f;
@@ -619,31 +689,14 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
           for (int r = 0; r < R; ++r) pair32_near<PSI, true>(D[r], sj, base + j == self[r], T[r]);
         }
       }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
     } else if constexpr (PSI) {
-      float D2[R][3], F[R][kNC + 1];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        for (int k = 0; k < 3; ++k) D2[r][k] = 2.0f * D[r][k];
-        for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0f;
-      }
-      for (int j = 0; j < kTile; ++j) {
-        const Src32 sj = ts[j];
-#pragma unroll
-        for (int r = 0; r < R; ++r) pair32_far<PSI>(D2[r], E[r], sj, F[r]);
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        T[r][0] = fmaf(D[r][0], F[r][0], F[r][1]);
-        T[r][1] = fmaf(D[r][1], F[r][0], F[r][2]);
-        T[r][2] = fmaf(D[r][2], F[r][0], F[r][3]);
-        T[r][3] = F[r][4];
-      }
+      far_tile_psi<R>(ts, D, E, fr, S);
     } else {
-      far_tile_sum<R>(ts, x, c, NX, D, T);
+      far_tile_sum<R>(ts, x, c, fr, NX, S);
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-      for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
   }
   store_partials<R>(a, blockIdx.y, idx, S);
 }
@@ -702,9 +755,10 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
       for (int k = 0; k < 3; ++k) NX[r][k] = -(float)x[r][k];
     }
     const Src32* ts = sh[buf];
-    float T[R][kNC];
+    const double4* fr = ctr + a.ntiles + 3 * (long)t;
     const bool has_self = wave_has_self<R>(self, base);
     if (has_self || __ballot(near) != 0) {
+      float T[R][kNC];
 #pragma unroll
       for (int r = 0; r < R; ++r)
         for (int k = 0; k < kNC; ++k) T[r][k] = 0.0f;
@@ -722,32 +776,14 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
           for (int r = 0; r < R; ++r) pair32_near<PSI, true>(D[r], sj, base + j == self[r], T[r]);
         }
       }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
     } else if constexpr (PSI) {
-      float D2[R][3], F[R][kNC + 1];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        for (int k = 0; k < 3; ++k) D2[r][k] = 2.0f * D[r][k];
-        for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0f;
-      }
-#pragma unroll 4
-      for (int j = 0; j < kTile; ++j) {
-        const Src32 sj = ts[j];
-#pragma unroll
-        for (int r = 0; r < R; ++r) pair32_far<PSI>(D2[r], E[r], sj, F[r]);
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        T[r][0] = fmaf(D[r][0], F[r][0], F[r][1]);
-        T[r][1] = fmaf(D[r][1], F[r][0], F[r][2]);
-        T[r][2] = fmaf(D[r][2], F[r][0], F[r][3]);
-        T[r][3] = F[r][4];
-      }
+      far_tile_psi<R>(ts, D, E, fr, S);
     } else {
-      far_tile_sum<R>(ts, x, c, NX, D, T);
+      far_tile_sum<R>(ts, x, c, fr, NX, S);
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-      for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
     __syncthreads();
   }
   store_partials<R>(a, by, idx, S);
@@ -1337,7 +1373,7 @@ BufferPlan plan_buffers(int nt, const Shape& sh) {
   BufferPlan p;
   p.src64 = ns_pad * sizeof(Src64);
   p.src32 = ns_pad * sizeof(Src32);
-  p.ctr = (size_t)sh.ntiles * sizeof(double4) + 256;
+  p.ctr = 4 * (size_t)sh.ntiles * sizeof(double4) + 256;  // centres, then FP32C tile frames
   p.trad = (size_t)sh.ntiles * sizeof(float) + 256;
   p.srcq = ns_pad * sizeof(double);
   p.part = (size_t)sh.splits * kNC * nt * sizeof(double);

@@ -311,6 +311,35 @@ def test_fp32c_variants_agree(omesh, lds, near):
     c.close()
 
 
+@pytest.mark.parametrize("drop,near", [(0, 2.0), (137, 2.0), (137, 4.0)])
+def test_fp32c_padded_tiles(omesh, drop, near):
+    """Source counts that leave the last tile partly padded (level 2: 320
+    sources; level 5 with `drop` faces removed): padding records add nothing
+    and never spoil the far form's shared reciprocals, in every launch shape."""
+    m = omesh(2 if drop == 0 else 5)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=3, nvort=16, sigma=0.2)
+    fz = fz[m.fmask == 0]
+    keep = np.ones(len(fz), bool)
+    keep[np.random.default_rng(5).choice(len(fz), drop, replace=False)] = False
+    assert keep.sum() % 256 != 0
+    c = L.Context(0)
+    c.set_state(lm.vx, vz, lm.fx[keep], fz[keep], lm.farea[keep], lm.fmask[keep])
+    c.set_near_factor(near)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
+    a = c.get_state()
+    for R in (1, 2, 3, 4):
+        c.set_launch(R, 0)
+        for psi in (False, True):
+            c.solve(L.PREC_FP32C, L.SCHEME_LAGR, want_psi=psi)
+            b = c.get_state()
+            assert np.all(np.isfinite(b["vu"])) and np.all(np.isfinite(b["fu"]))
+            assert max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"])) < 1e-6, (R, psi)
+            if psi:
+                assert max(normwise(b["vpsi"], a["vpsi"]), normwise(b["fpsi"], a["fpsi"])) < 1e-6, R
+    c.close()
+
+
 @pytest.mark.parametrize("lds", [True, False])
 def test_fp64_staging_variants(omesh, lds):
     """FP64 pair sums read through LDS or the scalar cache agree with the oracle."""
