@@ -32,6 +32,11 @@ import time
 
 import numpy as np
 
+# one hardware queue per lane stream of the serving sample (the runtime reads
+# this at the first HIP call; its default is 4)
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 

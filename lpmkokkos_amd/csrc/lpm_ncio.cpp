@@ -191,6 +191,12 @@ void Writer::var_copy(const std::string& name, Type type, const std::vector<int>
 }
 
 bool Writer::write(const std::string& path, std::string& err) const {
+  // a fixed dimension of length 0 would read as the record dimension
+  for (const auto& d : dims_)
+    if (d.second <= 0 || d.second > 0x7FFFFFFF) {
+      err = "dimension " + d.first + " has length " + std::to_string(d.second) + " (must be in [1, 2^31))";
+      return false;
+    }
   // the header twice: once to size it (begin offsets are fixed width)
   std::vector<int64_t> begins(vars_.size(), 0);
   Out h;

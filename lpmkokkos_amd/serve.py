@@ -122,28 +122,51 @@ def request_bytes(spec: MemberSpec) -> int:
     return state_bytes(nv, 20 * 4 ** spec.level)
 
 
+class _Lane:
+    """A batched lane: one ensemble stepped one RK4 step at a time."""
+
+    def __init__(self, device: int, lo: int, hi: int):
+        self.lo, self.hi = lo, hi  # levels served, inclusive
+        self.ens = Ensemble(device)
+        self.stream = self.ens.stream()
+        self.members: list[Request] = []
+        self.inflight = False
+
+
 class Server:
-    """Continuous batching in lanes.  Requests below `large_level` share the
-    batched lane: one ensemble, one persistent pair kernel per evaluation over
-    all of their work, stepped one RK4 step at a time so that membership can
-    change between steps.  A request at `large_level` or above runs in a lane of
-    its own (its context's stream, all its steps queued at once), so a short
-    request never waits for a large request's step; the GPU interleaves the
-    lanes' workgroups."""
+    """Continuous batching in lanes.  Requests below `large_level` are batched
+    by size class: each class (levels up to each of `batch_levels`) has its own
+    lane -- one ensemble, one persistent pair kernel per evaluation over all of
+    its members' work, stepped one RK4 step at a time so that membership can
+    change between steps -- so a small request is held to the steps of
+    requests of its own size, not to a level-7 step.  A request at
+    `large_level` or above runs in a large lane (one of `large_lanes` streams,
+    all its steps queued at once).  The GPU interleaves the lanes' workgroups.
+
+    Every lane's stream is created once, at construction: the runtime shares
+    its hardware queues (GPU_MAX_HW_QUEUES, 4 unless set before the first HIP
+    call) among all streams and runs a shared queue's work in order, so lanes
+    that share a queue wait for each other; `python -m lpmkokkos_amd.serve`
+    and bench.py ask for 8.  A large request that finds every large lane busy
+    waits for one; requests behind it are admitted."""
 
     def __init__(self, device: int = 0, budget_bytes: float = 200e9, max_active: int = 256, dt: float = 0.005,
                  omega: float = 2 * math.pi, prec: int = PREC_FP32C, keep_results: bool = False,
-                 log=None, large_level: int = 8):
+                 log=None, large_level: int = 8, large_lanes: int = 2, batch_levels=(5, 6, 7)):
         self.device = device
         self.budget = budget_bytes
         self.max_active = max_active
         self.dt, self.omega, self.prec = dt, omega, prec
         self.keep_results = keep_results
         self.large_level = large_level
-        self.ens = Ensemble(device)
-        self.ens_stream = self.ens.stream()
-        self.batch: list[Request] = []  # members of the batched lane
-        self.large: list[Request] = []  # requests in lanes of their own
+        bounds = sorted({b for b in batch_levels if b < large_level - 1} | {large_level - 1})
+        self.lanes = [_Lane(device, lo, hi) for lo, hi in zip([0] + [b + 1 for b in bounds[:-1]], bounds)]
+        # large lanes: empty ensembles used for their streams
+        self.lane_owners = [Ensemble(device) for _ in range(large_lanes)]
+        self.lane_streams = [e.stream() for e in self.lane_owners]
+        self.lane_of: dict[int, int] = {}  # request id -> large lane
+        self.lanes_max_busy = 0
+        self.large: list[Request] = []  # requests in large lanes
         # retired contexts not yet destroyed: freeing device memory waits for
         # the whole GPU, so it is done when nothing is in flight (or when an
         # admission needs the memory)
@@ -151,24 +174,45 @@ class Server:
         self.used = 0      # bytes of live requests
         self.retired_bytes = 0
         self.log = log or (lambda s: None)
+        queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        if len(self.lanes) + large_lanes + 1 > queues:
+            self.log(f"[serve] {len(self.lanes) + large_lanes} lane streams share {queues} hardware queues: "
+                     "lanes may wait for each other (set GPU_MAX_HW_QUEUES before the first HIP call)")
+
+    @property
+    def batch(self) -> list[Request]:
+        return [r for lane in self.lanes for r in lane.members]
 
     @property
     def active(self) -> list[Request]:
         return self.batch + self.large
 
+    def lane_for(self, level: int) -> _Lane:
+        return next(lane for lane in self.lanes if lane.lo <= level <= lane.hi)
+
     def warm(self, levels):
         """Build the device mesh cache for `levels` (once per process)."""
         for lev in sorted(set(levels)):
-            c = Context(self.device)
+            c = Context(self.device, stream=self.lanes[0].stream)
             c.set_state_level(lev, make_ic_zero())
             c.close()
+
+    def free_lane(self) -> int | None:
+        busy = set(self.lane_of.values())
+        return next((k for k in range(len(self.lane_streams)) if k not in busy), None)
 
     def _admit(self, r: Request, now: float):
         large = r.spec.level >= self.large_level
         t = time.perf_counter()
-        # batched-lane members queue on the ensemble's stream; a large request
-        # gets a stream of its own
-        stream = None if large else self.ens_stream
+        # a batched request queues on its lane's ensemble stream, a large
+        # request on a free large lane's
+        if large:
+            k = self.free_lane()
+            self.lane_of[r.rid] = k
+            stream = self.lane_streams[k]
+        else:
+            lane = self.lane_for(r.spec.level)
+            stream = lane.stream
         if r.checkpoint:
             r.ctx = Context(self.device, stream=stream)
             r.ctx.restore(r.checkpoint)
@@ -181,9 +225,23 @@ class Server:
         if large:
             r.ctx.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=r.steps - r.steps_done)
             self.large.append(r)
+            self.lanes_max_busy = max(self.lanes_max_busy, len(self.large))
         else:
-            self.ens.add(r.ctx)
-            self.batch.append(r)
+            lane.ens.add(r.ctx)
+            lane.members.append(r)
+
+    def _release(self, r: Request):
+        if r.spec.level >= self.large_level:
+            self.large.remove(r)
+            del self.lane_of[r.rid]
+        else:
+            lane = self.lane_for(r.spec.level)
+            lane.ens.remove(r.ctx)
+            lane.members.remove(r)
+        self.retired.append(r.ctx)
+        r.ctx = None
+        self.used -= r.bytes
+        self.retired_bytes += r.bytes
 
     def _retire(self, r: Request, now: float):
         if self.keep_results:
@@ -191,15 +249,7 @@ class Server:
         else:
             r.ctx.synchronize()
         r.finished = now
-        if r in self.batch:
-            self.ens.remove(r.ctx)
-            self.batch.remove(r)
-        else:
-            self.large.remove(r)
-        self.retired.append(r.ctx)
-        r.ctx = None
-        self.used -= r.bytes
-        self.retired_bytes += r.bytes
+        self._release(r)
 
     def _free_retired(self):
         for c in self.retired:
@@ -216,12 +266,7 @@ class Server:
             name = f"request_{r.rid}.nc"
             r.ctx.checkpoint(os.path.join(directory, name), time=r.steps_done * self.dt, omega=self.omega,
                              level=r.spec.level)
-            self.ens.remove(r.ctx)
-            self.batch.remove(r)
-            self.retired.append(r.ctx)
-            r.ctx = None
-            self.used -= r.bytes
-            self.retired_bytes += r.bytes
+            self._release(r)
             entries.append((r, name, 0.0))
         entries += [(r, None, max(r.arrival - now, 0.0)) for r in pending]
         doc = {"dt": self.dt, "omega": self.omega, "prec": self.prec,
@@ -235,8 +280,9 @@ class Server:
     def run(self, requests: list[Request], stop_after_steps: int | None = None,
             checkpoint_dir: str | None = None) -> dict:
         """Serve `requests` to completion, or, with stop_after_steps, stop after
-        that many batched steps: admit no more, let the large lanes finish, and
-        checkpoint what is left into checkpoint_dir (see load_manifest)."""
+        that many batched steps (summed over the lanes): admit no more, let
+        running steps and large lanes finish, and checkpoint what is left into
+        checkpoint_dir (see load_manifest)."""
         if stop_after_steps is not None and not checkpoint_dir:
             raise ValueError("stop_after_steps needs a checkpoint_dir")
         suspended = 0
@@ -250,63 +296,76 @@ class Server:
         busy = 0.0
         pairs = 0.0
         steps = 0
-        batch_inflight = False
         last_log = last = t0
         while pending or self.batch or self.large:
             tn = time.perf_counter()
-            if batch_inflight or self.large:
+            inflight = any(lane.inflight for lane in self.lanes)
+            if inflight or self.large:
                 busy += tn - last
             last = tn
             now = tn - t0
             # completions
-            if batch_inflight and self.ens.done():
-                batch_inflight = False
-                steps += 1
-                for r in list(self.batch):
-                    r.steps_done += 1
-                    inc = pairs_per_step(r.spec.level)
-                    r.pairs += inc
-                    pairs += inc
-                    if r.steps_done >= r.steps:
-                        self._retire(r, now)
-                        done.append(r)
+            for lane in self.lanes:
+                if lane.inflight and lane.ens.done():
+                    lane.inflight = False
+                    steps += 1
+                    for r in list(lane.members):
+                        r.steps_done += 1
+                        inc = pairs_per_step(r.spec.level)
+                        r.pairs += inc
+                        pairs += inc
+                        if r.steps_done >= r.steps:
+                            self._retire(r, now)
+                            done.append(r)
             for r in list(self.large):
                 if r.ctx.done():
+                    inc = pairs_per_step(r.spec.level) * (r.steps - r.steps_done)
                     r.steps_done = r.steps
-                    r.pairs = pairs_per_step(r.spec.level) * r.steps
-                    pairs += r.pairs
+                    r.pairs += inc
+                    pairs += inc
                     self._retire(r, now)
                     done.append(r)
-            if self.retired and not batch_inflight and not self.large:
+            inflight = any(lane.inflight for lane in self.lanes)
+            if self.retired and not inflight and not self.large:
                 self._free_retired()
-            if stop_after_steps is not None and steps >= stop_after_steps and not batch_inflight:
-                if not self.large:
+            if stop_after_steps is not None and steps >= stop_after_steps:
+                if not inflight and not self.large:
                     suspended = self._suspend(pending, now, checkpoint_dir)
                     pending = []
                     self._free_retired()
                     break
                 time.sleep(0.0005)
                 continue
-            # admission in arrival order while a slot and memory allow; the
-            # batched lane changes membership only between steps
-            while pending and pending[0].arrival <= now and len(self.active) < self.max_active:
-                need = request_bytes(pending[0].spec)
+            # admission in arrival order while a slot and memory allow; a lane
+            # changes membership only between its steps, and a large request
+            # needs a free large lane (later arrivals may pass a waiting one)
+            i = 0
+            while i < len(pending) and pending[i].arrival <= now and len(self.active) < self.max_active:
+                r = pending[i]
+                if r.spec.level >= self.large_level:
+                    if self.free_lane() is None:
+                        i += 1
+                        continue
+                elif self.lane_for(r.spec.level).inflight:
+                    i += 1
+                    continue
+                need = request_bytes(r.spec)
                 if self.used + self.retired_bytes + need > self.budget:
-                    if self.retired and self.used + need <= self.budget and not batch_inflight:
+                    if self.retired and self.used + need <= self.budget and not inflight:
                         self._free_retired()  # memory pressure: free now, even while a large lane runs
                     else:
                         break
-                if pending[0].spec.level < self.large_level and batch_inflight:
-                    break
-                self._admit(pending.pop(0), now)
-            if not batch_inflight and self.batch:
-                self.ens.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=1)
-                batch_inflight = True
+                self._admit(pending.pop(i), now)
+            for lane in self.lanes:
+                if not lane.inflight and lane.members:
+                    lane.ens.advance(self.dt, self.omega, SCHEME_LAGR, self.prec, nsteps=1)
+                    lane.inflight = True
+            inflight = any(lane.inflight for lane in self.lanes)
             if tn - last_log > 10.0:
                 last_log = tn
                 self.log(f"[serve] t={now:.0f}s done={len(done)} batched={len(self.batch)} large={len(self.large)} "
                          f"pending={len(pending)} batched steps={steps} memory={self.used / 1e9:.1f} GB")
-            if not batch_inflight and not self.large:
+            if not inflight and not self.large:
                 if pending:
                     time.sleep(min(max(pending[0].arrival - (time.perf_counter() - t0), 0.0), 0.05))
             else:
@@ -314,28 +373,36 @@ class Server:
         wall = time.perf_counter() - t0
         self._free_retired()
         lat = np.array([r.latency for r in done] or [0.0])
+        below = [r.latency for r in done if r.spec.level < self.large_level] or [0.0]
         return {
             "requests": len(done),
             "suspended": suspended,
             "wall_s": wall,
             "device_busy_s": busy,
             "batched_steps": steps,
+            "batch_lanes": [[lane.lo, lane.hi] for lane in self.lanes],
             "large_lane_requests": int(sum(1 for r in done if r.spec.level >= self.large_level)),
+            "large_lanes": len(self.lane_streams),
+            "large_lanes_max_busy": self.lanes_max_busy,
             "pairs": pairs,
             "pairs_per_s": pairs / wall,
             "pairs_per_busy_s": pairs / busy if busy else None,
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()),
-            "latency_p50_s_below_large": float(np.percentile([r.latency for r in done if r.spec.level < self.large_level]
-                                                             or [0.0], 50)),
+            "latency_p50_s_below_large": float(np.percentile(below, 50)),
+            "latency_p50_s_by_level": {int(l): float(np.percentile([r.latency for r in done if r.spec.level == l], 50))
+                                       for l in sorted({r.spec.level for r in done})},
             "admission_mean_s": float(np.mean([r.admit_s for r in done] or [0.0])),
             "admission_max_s": float(np.max([r.admit_s for r in done] or [0.0])),
             "levels": {int(l): int(sum(1 for r in done if r.spec.level == l)) for l in sorted({r.spec.level for r in done})},
         }
 
     def close(self):
-        self.ens.close()
+        for lane in self.lanes:
+            lane.ens.close()
+        for e in self.lane_owners:
+            e.close()
 
 
 def measure_service_times(levels, steps: int = 20, device: int = 0) -> dict:
@@ -364,6 +431,10 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--seed", type=int, default=4)
     args = ap.parse_args(argv)
+    # one hardware queue per lane stream (read at the first HIP call, below;
+    # the runtime's default is 4)
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     probs = {l: p for l, p in LEVEL_PROBS.items() if l <= args.max_level}
     tot = sum(probs.values())
     probs = {l: p / tot for l, p in probs.items()}

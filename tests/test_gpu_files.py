@@ -140,6 +140,11 @@ def test_checkpoint_without_tree(tmp_path):
     b = L.Context(0)
     with pytest.raises(L.LpmError):
         b.restore(str(tmp_path / "missing.nc"))
+    # a state without vertices: netCDF classic has no fixed dimension of length 0
+    m = L.TreeMesh(2)
+    b.set_state(np.zeros((0, 3)), np.zeros(0), m.fx, np.ones(m.nfaces), m.farea, m.fmask)
+    with pytest.raises(L.LpmError):
+        b.checkpoint(str(tmp_path / "z.nc"))
     a.close()
     b.close()
 

@@ -23,6 +23,7 @@ def test_requests_match_alone_under_admission_pressure():
     srv.close()
     assert res["requests"] == 10
     assert 0 < res["large_lane_requests"] < 10
+    assert res["large_lanes_max_busy"] <= res["large_lanes"] == 2
     assert res["latency_p99_s"] >= res["latency_p50_s"] > 0
     assert max(r.admitted - r.arrival for r in reqs) > 0  # some request queued
     for r in reqs:
@@ -42,6 +43,28 @@ def test_budget_rejects_oversized_request():
     srv.close()
 
 
+def test_large_requests_wait_for_a_lane():
+    """Five large requests (level 4 with large_level 4) and one large lane:
+    they run one after another on the lane's stream, short requests arriving
+    meanwhile are admitted past the waiting ones, and every result equals the
+    request run alone."""
+    big = [S.Request(rid=k, arrival=0.0, spec=E.MemberSpec(level=4, vortex_seed=30 + k), steps=4) for k in range(5)]
+    small = [S.Request(rid=5 + k, arrival=0.0, spec=E.MemberSpec(level=2, vortex_seed=40 + k), steps=4)
+             for k in range(3)]
+    srv = S.Server(keep_results=True, large_level=4, large_lanes=1)
+    res = srv.run(big + small)
+    srv.close()
+    assert res["requests"] == 8 and res["large_lanes_max_busy"] == 1
+    assert max(r.finished for r in small) < max(r.finished for r in big)
+    for r in big + small:
+        c = E.make_member(r.spec)
+        c.advance(0.005, 2 * math.pi, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=r.steps)
+        s = c.get_state()
+        for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
+            assert np.array_equal(s[k], r.result[k]), (r.rid, k)
+        c.close()
+
+
 def test_large_lane_does_not_hold_short_requests():
     """A level-7 request (about 3 s of device time) arrives first; level-2
     requests arriving while it runs finish in a small fraction of its time,
@@ -49,7 +72,7 @@ def test_large_lane_does_not_hold_short_requests():
     big = S.Request(rid=0, arrival=0.0, spec=E.MemberSpec(level=7, vortex_seed=5), steps=20)
     small = [S.Request(rid=1 + k, arrival=0.2 + 0.05 * k, spec=E.MemberSpec(level=2, vortex_seed=10 + k), steps=20)
              for k in range(8)]
-    srv = S.Server(large_level=7)
+    srv = S.Server(large_level=7, batch_levels=(), large_lanes=1)  # two streams: distinct hardware queues
     res = srv.run([big] + small)
     srv.close()
     assert res["requests"] == 9 and res["large_lane_requests"] == 1
@@ -83,4 +106,25 @@ def test_suspend_and_resume_match_alone(tmp_path):
         s = c.get_state()
         for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
             assert np.array_equal(s[k], r.result[k]), (rid, k)
+        c.close()
+
+
+def test_size_class_lanes_step_independently():
+    """Batched lanes by size class: level-2 requests in the (0..5) lane finish
+    while a level-6 request in the (6..6) lane is still stepping, and every
+    result equals the request run alone."""
+    mid = S.Request(rid=0, arrival=0.0, spec=E.MemberSpec(level=6, vortex_seed=50), steps=6)
+    small = [S.Request(rid=1 + k, arrival=0.0, spec=E.MemberSpec(level=2, vortex_seed=60 + k), steps=6)
+             for k in range(4)]
+    srv = S.Server(keep_results=True, large_level=7, batch_levels=(5,), large_lanes=0)
+    res = srv.run([mid] + small)
+    srv.close()
+    assert res["batch_lanes"] == [[0, 5], [6, 6]] and res["requests"] == 5
+    assert max(r.finished for r in small) < mid.finished
+    for r in [mid] + small:
+        c = E.make_member(r.spec)
+        c.advance(0.005, 2 * math.pi, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=r.steps)
+        s = c.get_state()
+        for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
+            assert np.array_equal(s[k], r.result[k]), (r.rid, k)
         c.close()
