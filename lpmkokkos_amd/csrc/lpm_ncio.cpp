@@ -319,7 +319,20 @@ bool Reader::open(const std::string& path, std::string& err) {
         v.begin = offset64 ? (int64_t)in.u64() : (int64_t)in.u32();
         vars_[v.name] = v;
       }
-    if (!in.bad) return true;
+    if (!in.bad) {
+      // every variable's data must lie inside the file
+      f.clear();
+      f.seekg(0, std::ios::end);
+      const int64_t size = (int64_t)f.tellg();
+      for (const auto& kv : vars_) {
+        const VarInfo& v = kv.second;
+        if (v.begin < 0 || v.count < 0 || v.begin + v.count * type_size(v.type) > size) {
+          err = path + ": truncated (variable " + v.name + " ends past the end of the file)";
+          return false;
+        }
+      }
+      return true;
+    }
     if ((size_t)f.gcount() < want) {
       err = path + ": truncated or malformed header";
       return false;

@@ -158,3 +158,38 @@ def test_errors(tmp_path):
     g.close()
     with pytest.raises(L.LpmError):
         L.TreeMesh.from_nc(str(tmp_path / "other.nc"))
+
+
+def test_reads_files_written_by_another_writer(files, tmp_path):
+    """A mesh file written by scipy's netCDF writer (variables in another order,
+    extra dimensions, variables and attributes of other types) reads back the
+    same arrays: the reader follows the format, not this library's layout."""
+    f, v = read(files[3])
+    path = str(tmp_path / "other_writer.nc")
+    g = scipy.io.netcdf_file(path, "w", version=2)
+    g.title = "written by scipy"
+    g.history = np.array([1.5, 2.5])
+    g.createDimension("unused", 7)
+    for name, length in f.dimensions.items():
+        g.createDimension(name, length)
+    g.createVariable("noise", "f", ("unused",))[:] = np.arange(7, dtype=np.float32)
+    for name in reversed(sorted(f.variables)):
+        src = f.variables[name]
+        var = g.createVariable(name, src.typecode(), src.dimensions)
+        var[:] = v[name]
+        var.long_name = name.replace("_", " ")
+    g.close()
+    m, r = L.TreeMesh(3), L.TreeMesh.from_nc(path)
+    assert (r.nverts, r.nedges, r.nfaces, r.nleaves) == (m.nverts, m.nedges, m.nfaces, m.nleaves)
+    for k in ("vx", "vlag", "fx", "flag", "farea", "fmask", "fverts", "fparent"):
+        assert np.array_equal(getattr(r, k), getattr(m, k)), k
+
+
+def test_truncated_files_fail_cleanly(files, tmp_path):
+    """Every truncation of a mesh file is reported as an error, never read as data."""
+    data = open(files[1], "rb").read()
+    for cut in (0, 3, 4, 40, 200, len(data) // 2, len(data) - 1):
+        p = tmp_path / f"cut_{cut}.nc"
+        p.write_bytes(data[:cut])
+        with pytest.raises(L.LpmError):
+            L.TreeMesh.from_nc(str(p))
