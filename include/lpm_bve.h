@@ -243,7 +243,7 @@ int lpm_bve_write_vtk(lpm_bve_ctx* ctx, const char* path, int level, double omeg
 /* A caller's array for lpm_bve_write_vtk_arrays: at vertices [nverts][ncomp]
  * or at faces [nfaces][ncomp] in tree order (leaf rows are written). */
 typedef struct {
-  const char* name; /* no blanks or XML markup */
+  const char* name; /* 1-255 bytes; written as VTK encodes names (%XX in legacy files) */
   int ncomp;        /* 1..9 */
   int on_faces;     /* 0: point data, 1: cell data */
   const double* data;

@@ -34,6 +34,7 @@ namespace Lpm {
 
 typedef double Real;
 typedef int64_t Index;
+typedef short Short;
 static constexpr Real PI = 3.1415926535897932384626433832795027975;
 
 inline void check(int rc, const char* what) {
@@ -135,6 +136,11 @@ class BVESphere {
   Real Omega = 2 * PI;
   Real t = 0;
 
+  // passive tracers (src/LpmBVESphere.cpp:36-58): host arrays [nmaxverts] and
+  // [nmaxfaces], written with the state by outputVtk
+  std::vector<std::string> tracer_names;
+  std::vector<std::vector<Real>> tracer_verts, tracer_faces;
+
   // host mirrors (filled by treeInit / init_vorticity / updateHost)
   std::vector<Real> vert_x, vert_lag, face_x, face_lag, face_area;
   std::vector<uint8_t> face_mask;
@@ -144,7 +150,6 @@ class BVESphere {
   BVESphere(Index nmaxverts, Index nmaxedges, Index nmaxfaces, int nq = 0, int device = 0)
       : nmaxv_(nmaxverts), nmaxf_(nmaxfaces) {
     (void)nmaxedges;
-    (void)nq;
     physVerts.crds = {this, VX};
     physFaces.crds = {this, FX};
     faces.area = {this, FAREA};
@@ -154,6 +159,15 @@ class BVESphere {
     velocityVerts = {this, VU};
     velocityFaces = {this, FU};
     check(lpm_bve_create(device, &ctx_), "lpm_bve_create");
+    for (int k = 0; k < nq; ++k) create_tracer("tracer" + std::to_string(k));
+  }
+
+  // BVESphere::create_tracer: a new zero tracer; returns its index.
+  Short create_tracer(const std::string& name) {
+    tracer_names.push_back(name);
+    tracer_verts.emplace_back(nmaxv_, 0.0);
+    tracer_faces.emplace_back(nmaxf_, 0.0);
+    return (Short)(tracer_names.size() - 1);
   }
   ~BVESphere() { lpm_bve_destroy(ctx_); }
   BVESphere(const BVESphere&) = delete;
@@ -235,11 +249,15 @@ class BVESphere {
 
   // BVESphere::outputVtk (src/LpmBVESphere.cpp:90-113): legacy VTK polydata of
   // the leaf faces; a name ending in .vtp writes the XML layout of
-  // Polymesh2dVtkInterface::write with addFieldsToVtk.  `extra` fields follow
-  // the state's own.
+  // Polymesh2dVtkInterface::write with addFieldsToVtk.  The tracers follow the
+  // state's fields, and `extra` fields follow them.
   void outputVtk(const std::string& fname, const std::vector<VtkArray>& extra = {}) {
     const bool vtp = fname.size() > 4 && fname.compare(fname.size() - 4, 4, ".vtp") == 0;
     std::vector<lpm_vtk_array> a;
+    for (size_t k = 0; k < tracer_names.size(); ++k) {
+      a.push_back({tracer_names[k].c_str(), 1, 0, tracer_verts[k].data()});
+      a.push_back({tracer_names[k].c_str(), 1, 1, tracer_faces[k].data()});
+    }
     for (const VtkArray& x : extra) a.push_back({x.name.c_str(), x.ncomp, x.on_faces ? 1 : 0, x.data.data()});
     check(lpm_bve_write_vtk_arrays(ctx_, fname.c_str(), depth_, Omega, vtp ? LPM_VTK_XML_BINARY : LPM_VTK_LEGACY_ASCII,
                                    a.data(), (int)a.size()),

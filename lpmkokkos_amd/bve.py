@@ -365,6 +365,20 @@ class BVESphere:
         self.velocityVerts = DeviceView(self, "vu")
         self.velocityFaces = DeviceView(self, "fu")
         self.host = {}
+        # passive tracers (src/LpmBVESphere.cpp:36-58): host arrays carried by the
+        # particles, written with the state by outputVtk
+        self.tracer_names: list[str] = []
+        self.tracer_verts: list[np.ndarray] = []
+        self.tracer_faces: list[np.ndarray] = []
+        for k in range(nq):
+            self.create_tracer(f"tracer{k}")
+
+    def create_tracer(self, name: str) -> int:
+        """BVESphere::create_tracer: a new zero tracer named `name`; returns its index."""
+        self.tracer_names.append(name)
+        self.tracer_verts.append(np.zeros(self.nmax[0]))
+        self.tracer_faces.append(np.zeros(self.nmax[2]))
+        return len(self.tracer_names) - 1
 
     # PolyMesh2d::treeInit
     def treeInit(self, depth: int, seed: MeshSeed):
@@ -409,10 +423,15 @@ class BVESphere:
     def outputVtk(self, fname: str, arrays=None):
         """BVESphere::outputVtk (src/LpmBVESphere.cpp:90-113): legacy VTK polydata
         of the leaf faces; a .vtp name writes the XML layout of
-        Polymesh2dVtkInterface::write with addFieldsToVtk instead.  arrays: more
-        fields, as Context.write_vtk takes them."""
+        Polymesh2dVtkInterface::write with addFieldsToVtk instead.  The tracers
+        follow the state's fields; arrays: more fields after them, as
+        Context.write_vtk takes them."""
         fmt = _lib.VTK_XML_BINARY if fname.endswith(".vtp") else _lib.VTK_LEGACY_ASCII
-        self.ctx.write_vtk(fname, self.Omega, self.mesh.level, fmt, arrays)
+        nv, nf = self.mesh.nverts, self.mesh.nfaces
+        tracers = []
+        for name, tv, tf in zip(self.tracer_names, self.tracer_verts, self.tracer_faces):
+            tracers += [(name, False, tv[:nv]), (name, True, tf[:nf])]
+        self.ctx.write_vtk(fname, self.Omega, self.mesh.level, fmt, tracers + list(arrays or []))
 
     def checkpoint(self, fname: str):
         """The mesh, fields, time and rotation rate in one netCDF file."""

@@ -9,7 +9,7 @@
 // (case_name + "tri" + depth + "_dt" + dt + "_" + 4-digit step + ".vtp",
 // tests/LpmBVETest.cpp:105,236-237); for solid-body rotation the velocity and
 // position errors are added as vertex_/face_velocity_error and
-// vertex_/face_position_error (:67-70, :180-217).
+// vertex_/face_position_error (:67-70, :180-217), after the tracer "u dot x".
 //
 // Solid-body rotation (default) has an exact solution: velocity Omega z x x
 // and rigid rotation of the Lagrangian coordinates; the driver prints the
@@ -150,9 +150,18 @@ int main(int argc, char* argv[]) {
       phase0 = rh_phase(sphere.face_x, sphere.rel_vort_faces, sphere.face_mask, 4, 0.6);
     }
     const std::vector<Real> lag_verts = sphere.vert_lag;
+    // the reference driver's first tracer: u.x, zero for a tangent velocity
+    // (tests/LpmBVETest.cpp:60-64)
+    const Short dotprod = sphere.create_tracer("u dot x");
     // XML VTK output of the current (host-mirrored) state
     auto write_output = [&](int step) {
       if (case_name.empty()) return;
+      auto dot = [](const std::vector<Real>& x, const std::vector<Real>& u, std::vector<Real>& out) {
+        for (size_t i = 0; i < x.size() / 3; ++i)
+          out[i] = x[3 * i] * u[3 * i] + x[3 * i + 1] * u[3 * i + 1] + x[3 * i + 2] * u[3 * i + 2];
+      };
+      dot(sphere.vert_x, sphere.vel_verts, sphere.tracer_verts[dotprod]);
+      dot(sphere.face_x, sphere.vel_faces, sphere.tracer_faces[dotprod]);
       std::vector<BVESphere<IcosTriSphereSeed>::VtkArray> extra;
       if (!rh) {
         const Real c = std::cos(Omega * sphere.t), s = std::sin(Omega * sphere.t);

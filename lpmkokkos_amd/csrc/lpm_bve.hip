@@ -2397,9 +2397,9 @@ int lpm_bve_write_vtk_arrays(lpm_bve_ctx* c, const char* path, int level, double
   std::vector<VtkExtra> extra;
   for (int k = 0; k < narrays; ++k) {
     const lpm_vtk_array& a = arrays[k];
-    if (!a.name || !a.name[0] || std::strpbrk(a.name, " \t\n\r\"<>&") || a.ncomp < 1 || a.ncomp > 9 || !a.data)
+    if (!a.name || !a.name[0] || std::strlen(a.name) > 255 || a.ncomp < 1 || a.ncomp > 9 || !a.data)
       return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: array " + std::to_string(k) +
-                                   ": name must be non-empty without blanks or markup, 1 <= ncomp <= 9, data non-null");
+                                   ": name must have 1 to 255 characters, 1 <= ncomp <= 9, data non-null");
     extra.push_back({a.name, a.ncomp, a.on_faces != 0, a.data});
   }
   LPM_HIP(hipSetDevice(c->device));

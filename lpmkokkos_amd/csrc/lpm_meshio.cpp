@@ -218,6 +218,38 @@ struct Array {
   const double* data;  // n tuples
 };
 
+// Array names as the VTK writers store them: legacy files write blanks,
+// control characters, '%' and non-ASCII bytes as %XX (the names are
+// blank-separated tokens there); XML attributes escape the markup characters.
+std::string legacy_name(const std::string& n) {
+  std::string out;
+  char buf[4];
+  for (unsigned char ch : n) {
+    if (ch <= ' ' || ch > '~' || ch == '%') {
+      std::snprintf(buf, sizeof buf, "%%%02X", ch);
+      out += buf;
+    } else {
+      out += (char)ch;
+    }
+  }
+  return out;
+}
+
+std::string xml_name(const std::string& n) {
+  std::string out;
+  for (char ch : n) {
+    switch (ch) {
+      case '&': out += "&amp;"; break;
+      case '<': out += "&lt;"; break;
+      case '>': out += "&gt;"; break;
+      case '"': out += "&quot;"; break;
+      case '\'': out += "&apos;"; break;
+      default: out += ch;
+    }
+  }
+  return out;
+}
+
 }  // namespace
 
 bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& m, int format, std::string& err,
@@ -290,7 +322,7 @@ bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& 
     auto fields = [&](const std::vector<Array>& as, int64_t n) {
       f << "FIELD FieldData " << as.size() << '\n';
       for (const Array& a : as) {
-        f << a.name << ' ' << a.ncomp << ' ' << n << " double\n";
+        f << legacy_name(a.name) << ' ' << a.ncomp << ' ' << n << " double\n";
         o.doubles(a.data, n * a.ncomp, a.ncomp == 1 ? 9 : a.ncomp);
       }
     };
@@ -307,7 +339,7 @@ bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& 
     auto arrays = [&](const char* tag, const std::vector<Array>& as, int64_t n) {
       f << '<' << tag << ">\n";
       for (const Array& a : as) {
-        f << "<DataArray type=\"Float64\" Name=\"" << a.name << "\" NumberOfComponents=\"" << a.ncomp
+        f << "<DataArray type=\"Float64\" Name=\"" << xml_name(a.name) << "\" NumberOfComponents=\"" << a.ncomp
           << "\" format=\"" << fmt << "\">\n";
         o.doubles(a.data, n * a.ncomp, a.ncomp == 1 ? 9 : a.ncomp);
         f << "</DataArray>\n";

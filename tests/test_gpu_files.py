@@ -175,6 +175,12 @@ class _Cursor:
         return np.array(out, dtype=dtype)
 
 
+def legacy_decode(name):
+    """VTK legacy array names carry blanks and other bytes as %XX."""
+    import re
+    return re.sub(r"%([0-9A-F]{2})", lambda m: chr(int(m.group(1), 16)), name)
+
+
 def read_legacy(path):
     c = _Cursor(open(path, "rb").read())
     assert c.line().startswith("# vtk DataFile")
@@ -194,7 +200,7 @@ def read_legacy(path):
         _, _, narr = c.line().split()
         for _ in range(int(narr)):
             name, ncomp, nt, _ = c.line().split()
-            out[key][name] = c.values(int(ncomp) * int(nt), binary, np.float64).reshape(int(nt), int(ncomp))
+            out[key][legacy_decode(name)] = c.values(int(ncomp) * int(nt), binary, np.float64).reshape(int(nt), int(ncomp))
     return out
 
 
@@ -305,7 +311,7 @@ def test_vtk_caller_arrays(tmp_path):
         assert np.array_equal(d["point"]["tracer"].ravel(), sv)
         assert np.array_equal(d["cell"]["face_velocity_error"], pf[leaf])
     with pytest.raises(L.LpmError):
-        a.write_vtk(str(tmp_path / "y.vtk"), TWO_PI, arrays=[("bad name", False, sv)])
+        a.write_vtk(str(tmp_path / "y.vtk"), TWO_PI, arrays=[("", False, sv)])
     with pytest.raises(ValueError):
         a.write_vtk(str(tmp_path / "y.vtk"), TWO_PI, arrays=[("short", True, sv)])
     a.close()
@@ -325,10 +331,37 @@ def test_cpp_driver_writes_the_reference_files(tmp_path):
     names = sorted(p.name for p in tmp_path.iterdir())
     assert names == ["bve_testtri3_dt0.01_0000.vtp", "bve_testtri3_dt0.01_0002.vtp", "bve_testtri3_dt0.01_0004.vtp"]
     d0 = read_xml(str(tmp_path / names[0]))
-    assert list(d0["point"]) == ["relvort", "absvort", "stream_fn", "velocity", "vertex_velocity_error",
+    assert list(d0["point"]) == ["relvort", "absvort", "stream_fn", "velocity", "u dot x", "vertex_velocity_error",
                                  "vertex_position_error"]
-    assert list(d0["cell"]) == ["area", "relvort", "absvort", "stream_fn", "velocity", "face_velocity_error",
-                                "face_position_error"]
+    assert list(d0["cell"]) == ["area", "relvort", "absvort", "stream_fn", "velocity", "u dot x",
+                                "face_velocity_error", "face_position_error"]
+    assert np.abs(d0["point"]["u dot x"]).max() < 1e-12 and np.abs(d0["cell"]["u dot x"]).max() < 1e-12
     assert np.all(d0["point"]["vertex_position_error"] == 0) and np.abs(d0["cell"]["face_velocity_error"]).max() < 0.1
     d2 = read_xml(str(tmp_path / names[2]))
     assert 0 < np.abs(d2["cell"]["face_position_error"]).max() < 0.01
+
+
+def test_sphere_tracers_in_vtk(tmp_path):
+    """BVESphere tracers (create_tracer, and the nq built-in ones) are written
+    after the state's fields, leaves only for faces, under their names -- blanks
+    and markup encoded as VTK encodes them (%XX in legacy files, entities in XML)."""
+    lev = 2
+    nv, ne, nf = L.MeshSeed().setMaxAllocations(lev)
+    sphere = L.BVESphere(nv, ne, nf, nq=1)
+    sphere.treeInit(lev, L.MeshSeed())
+    sphere.init_vorticity(field())
+    k = sphere.create_tracer('u dot x <"%">')
+    assert sphere.tracer_names == ["tracer0", 'u dot x <"%">'] and k == 1
+    sphere.tracer_verts[0][:nv] = np.arange(nv)
+    sphere.tracer_faces[1][:nf] = -np.arange(nf)
+    leaf = sphere.mesh.fmask == 0
+    for name in ("t.vtk", "t.vtp"):
+        sphere.outputVtk(str(tmp_path / name))
+        raw = open(tmp_path / name, "rb").read()
+        d = read_xml(str(tmp_path / name)) if name.endswith(".vtp") else read_legacy(str(tmp_path / name))
+        assert list(d["point"])[4:] == ["tracer0", 'u dot x <"%">']
+        assert list(d["cell"])[5:] == ["tracer0", 'u dot x <"%">']
+        assert np.array_equal(d["point"]["tracer0"].ravel(), np.arange(nv))
+        assert np.array_equal(d["cell"]['u dot x <"%">'].ravel(), -np.arange(nf)[leaf])
+        assert (b"u%20dot%20x%20<\"%25\">" in raw) if name.endswith(".vtk") else (b"u dot x &lt;&quot;%&quot;&gt;" in raw)
+
