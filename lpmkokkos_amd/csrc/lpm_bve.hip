@@ -691,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
       }
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
+        for (int k = 0; k < (PSI ? kNC : 3); ++k) S[r][k] += (double)T[r][k];
     } else if constexpr (PSI) {
       far_tile_psi<R>(ts, D, E, fr, S);
     } else {
@@ -778,7 +778,7 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
       }
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        for (int k = 0; k < kNC; ++k) S[r][k] += (double)T[r][k];
+        for (int k = 0; k < (PSI ? kNC : 3); ++k) S[r][k] += (double)T[r][k];
     } else if constexpr (PSI) {
       far_tile_psi<R>(ts, D, E, fr, S);
     } else {
