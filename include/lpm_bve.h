@@ -143,6 +143,14 @@ int lpm_bve_vertex_streamfn(double* psi, const double* vx, int64_t nv, const dou
                             const double* farea, const uint8_t* fmask, int64_t nf, void* stream);
 int lpm_bve_face_streamfn(double* psi, const double* fx, const double* fzeta, const double* farea,
                           const uint8_t* fmask, int64_t nf, void* stream);
+/* BVEVertexSolve / BVEFaceSolve (src/LpmBVEKernels.hpp:110-138, 236-261): the
+ * stream function and the velocity together, in one pass over the sources
+ * (the reference runs the two reductions one after the other); argument
+ * order as the reference's constructors: psi, u, targets, sources, ... */
+int lpm_bve_vertex_solve(double* psi, double* u, const double* vx, int64_t nv, const double* fx,
+                         const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, void* stream);
+int lpm_bve_face_solve(double* psi, double* u, const double* fx, const double* fzeta, const double* farea,
+                       const uint8_t* fmask, int64_t nf, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Solver context: the BVESphere fields and BVERK4 stage buffers of    */

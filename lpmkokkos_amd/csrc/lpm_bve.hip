@@ -1921,6 +1921,16 @@ int lpm_bve_face_streamfn(double* psi, const double* fx, const double* fzeta, co
                           const uint8_t* fmask, int64_t nf, void* stream) {
   return functor_eval(nullptr, psi, fx, nf, fx, fzeta, farea, fmask, nf, true, stream);
 }
+int lpm_bve_vertex_solve(double* psi, double* u, const double* vx, int64_t nv, const double* fx,
+                         const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, void* stream) {
+  if (!psi || !u) return fail(LPM_ERR_ARG, "lpm_bve_vertex_solve: null output");
+  return functor_eval(u, psi, vx, nv, fx, fzeta, farea, fmask, nf, false, stream);
+}
+int lpm_bve_face_solve(double* psi, double* u, const double* fx, const double* fzeta, const double* farea,
+                       const uint8_t* fmask, int64_t nf, void* stream) {
+  if (!psi || !u) return fail(LPM_ERR_ARG, "lpm_bve_face_solve: null output");
+  return functor_eval(u, psi, fx, nf, fx, fzeta, farea, fmask, nf, true, stream);
+}
 
 int lpm_bve_create(int device, lpm_bve_ctx** out) {
   if (!out) return fail(LPM_ERR_ARG, "lpm_bve_create: null output");

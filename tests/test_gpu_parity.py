@@ -25,8 +25,8 @@ def make_ctx(lm, vz, fz):
 
 
 def test_functor_entry_points(omesh):
-    """BVE{Vertex,Face}{Velocity,StreamFn} on caller-owned device arrays over all
-    tree faces (masked loop), against the oracle at vertices and active faces."""
+    """BVE{Vertex,Face}{Velocity,StreamFn,Solve} on caller-owned device arrays over
+    all tree faces (masked loop), against the oracle at vertices and active faces."""
     from helpers import DeviceArray
     m = omesh(4)
     fz = rh_field(m.fx)
@@ -49,7 +49,16 @@ def test_functor_entry_points(omesh):
     assert normwise(uf.numpy()[lv], of[lv]) < 1e-13
     assert normwise(pv.numpy(), opv) < 1e-13
     assert normwise(pf.numpy()[lv], opf[lv]) < 1e-13
-    for d in (vx, fx, z, a, msk, uv, uf, pv, pf):
+    # BVEVertexSolve / BVEFaceSolve: both fields in one pass, the same sums
+    uv2, uf2 = DeviceArray.zeros(m.vx.shape), DeviceArray.zeros(m.fx.shape)
+    pv2, pf2 = DeviceArray.zeros(m.nv), DeviceArray.zeros(m.nf)
+    L._lib.call("lpm_bve_vertex_solve", P(pv2), P(uv2), P(vx), m.nv, P(fx), P(z), P(a), P(msk), m.nf, None)
+    L._lib.call("lpm_bve_face_solve", P(pf2), P(uf2), P(fx), P(z), P(a), P(msk), m.nf, None)
+    assert np.array_equal(uv2.numpy(), uv.numpy()) and np.array_equal(pv2.numpy(), pv.numpy())
+    assert np.array_equal(uf2.numpy()[lv], uf.numpy()[lv]) and np.array_equal(pf2.numpy()[lv], pf.numpy()[lv])
+    with pytest.raises(L.LpmError):
+        L._lib.call("lpm_bve_face_solve", None, P(uf2), P(fx), P(z), P(a), P(msk), m.nf, None)
+    for d in (vx, fx, z, a, msk, uv, uf, pv, pf, uv2, uf2, pv2, pf2):
         d.free()
 
 
