@@ -85,6 +85,11 @@ int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vla
  * baseTreeDepth.  A netCDF classic (64-bit offset) file. */
 int lpm_mesh_write_nc(int level, const char* seed_path, const char* path);
 
+/* PolyMesh2d::outputVtk (src/LpmPolyMesh2d.cpp:79-85) of the treeInit mesh:
+ * vertices, leaf faces and their areas ("area" cell data), in one of the
+ * lpm_vtk_format layouts. */
+int lpm_mesh_write_vtk(int level, const char* seed_path, const char* path, int format);
+
 /* PolyMeshReader (src/LpmNetCDF.cpp:14-240): the particle arrays of a mesh
  * file written by lpm_mesh_write_nc or lpm_bve_checkpoint, in the layout of
  * lpm_icos_tree_init.  Call with NULL arrays to get counts {nverts, nedges,

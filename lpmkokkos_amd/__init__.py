@@ -7,4 +7,4 @@ from ._lib import (PREC_FP32C, PREC_FP64, SCHEME_LAGR, SCHEME_REF, VTK_LEGACY_AS
                    VTK_XML_ASCII, VTK_XML_BINARY, LpmError, lib)
 from .bve import (BVERK4, BVESphere, Context, IcosTriSphereSeed, MeshSeed, RossbyHaurwitzWave,  # noqa: F401
                   SolidBodyRotation, SpherePoisson, TreeMesh, VorticityInitialCondition, area_mean, gauss_params,
-                  poisson_y54, write_mesh_nc)
+                  poisson_y54, write_mesh_nc, write_mesh_vtk)

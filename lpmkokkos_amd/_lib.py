@@ -53,6 +53,7 @@ SIGNATURES = {
     "lpm_area_mean": [_dp, _dp, _bp, C.c_int64, C.POINTER(C.c_double)],
     "lpm_poisson_y54": [_dp, C.c_int64, _dpn, _dpn],
     "lpm_mesh_write_nc": [C.c_int, C.c_char_p, C.c_char_p],
+    "lpm_mesh_write_vtk": [C.c_int, C.c_char_p, C.c_char_p, C.c_int],
     "lpm_mesh_read_nc": [C.c_char_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
     "lpm_rng_draws": [C.c_uint64, C.c_int64, _dpn, C.c_int64, _dpn],
     "lpm_bve_vertex_velocity": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

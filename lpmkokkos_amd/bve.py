@@ -113,6 +113,13 @@ class TreeMesh:
         return m
 
 
+def write_mesh_vtk(level: int, path: str, fmt: int = _lib.VTK_LEGACY_ASCII, seed: MeshSeed | None = None):
+    """PolyMesh2d::outputVtk (src/LpmPolyMesh2d.cpp:79-85) of the level-`level`
+    treeInit mesh: vertices, leaf faces and their areas."""
+    seed = seed or MeshSeed()
+    call("lpm_mesh_write_vtk", int(level), seed.path.encode() if seed.path else None, path.encode(), int(fmt))
+
+
 def write_mesh_nc(level: int, path: str, seed: MeshSeed | None = None):
     """NcWriter::writePolymesh (src/LpmNetCDF_Impl.hpp:46-218) of the level-`level`
     treeInit mesh, as a netCDF classic (64-bit offset) file."""

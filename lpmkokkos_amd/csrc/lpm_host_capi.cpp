@@ -98,6 +98,28 @@ int lpm_mesh_write_nc(int level, const char* seed_path, const char* path) {
   return ok();
 }
 
+int lpm_mesh_write_vtk(int level, const char* seed_path, const char* path, int format) {
+  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: level must be in [0, 12]");
+  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: null path");
+  if (format < LPM_VTK_LEGACY_ASCII || format > LPM_VTK_XML_BINARY)
+    return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: unknown format");
+  try {
+    SeedTables file_seed;
+    const SeedTables* seed = &builtin_icos_seed();
+    if (seed_path) {
+      std::string err;
+      if (!read_icos_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_vtk: " + err);
+      seed = &file_seed;
+    }
+    const TreeMesh m = build_icos_tree(level, *seed);
+    std::string err;
+    if (!write_mesh_vtk(path, m, format, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_vtk: " + err);
+  } catch (const std::exception& e) {
+    return fail(LPM_ERR_NOMEM, std::string("lpm_mesh_write_vtk: ") + e.what());
+  }
+  return ok();
+}
+
 int lpm_mesh_read_nc(const char* path, double* vx, double* vlag, double* fx, double* flag, double* farea,
                      uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts) {
   if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_read_nc: null path");

@@ -252,21 +252,109 @@ std::string xml_name(const std::string& n) {
 
 }  // namespace
 
+namespace {
+
+// Leaf faces of the tree (no kids, src/LpmVtkIO.cpp:23-30).
+std::vector<int64_t> leaf_faces(const TreeMesh& m) {
+  std::vector<int64_t> leaves;
+  for (int64_t f = 0; f < m.nfaces(); ++f)
+    if (m.face_kids[(size_t)(4 * f)] < 0) leaves.push_back(f);
+  return leaves;
+}
+
+// Polydata of points vx[nv][3] and the listed leaf faces, with point and cell
+// arrays (cell arrays already restricted to the leaves).
+bool write_polydata(const std::string& path, int format, int64_t nv, const double* vx, const TreeMesh& m,
+                    const std::vector<int64_t>& leaves, const std::vector<Array>& pdata,
+                    const std::vector<Array>& cdata, const char* title, std::string& err) {
+  if (format < kVtkLegacyAscii || format > kVtkXmlBinary) {
+    err = "unknown VTK format";
+    return false;
+  }
+  const bool xml = format >= kVtkXmlAscii, binary = format == kVtkLegacyBinary || format == kVtkXmlBinary;
+  const int64_t nc = (int64_t)leaves.size();
+  std::ofstream f(path, std::ios::binary | std::ios::trunc);
+  if (!f) {
+    err = "cannot open " + path + " for writing";
+    return false;
+  }
+  VtkOut o{f, binary, xml};
+  if (!xml) {
+    f << "# vtk DataFile Version 3.0\n" << title << '\n' << (binary ? "BINARY" : "ASCII")
+      << "\nDATASET POLYDATA\nPOINTS " << nv << " double\n";
+    o.doubles(vx, 3 * nv, 3);
+    f << "POLYGONS " << nc << ' ' << 4 * nc << '\n';
+    std::vector<int32_t> conn((size_t)(4 * nc));
+    for (int64_t c = 0; c < nc; ++c) {
+      conn[(size_t)(4 * c)] = 3;
+      for (int k = 0; k < 3; ++k) conn[(size_t)(4 * c + 1 + k)] = m.face_verts[(size_t)(3 * leaves[(size_t)c] + k)];
+    }
+    o.values(conn.data(), (int64_t)conn.size(), 4, "%d");
+    auto fields = [&](const std::vector<Array>& as, int64_t n) {
+      f << "FIELD FieldData " << as.size() << '\n';
+      for (const Array& a : as) {
+        f << legacy_name(a.name) << ' ' << a.ncomp << ' ' << n << " double\n";
+        o.doubles(a.data, n * a.ncomp, a.ncomp == 1 ? 9 : a.ncomp);
+      }
+    };
+    if (!pdata.empty()) {
+      f << "POINT_DATA " << nv << '\n';
+      fields(pdata, nv);
+    }
+    if (!cdata.empty()) {
+      f << "CELL_DATA " << nc << '\n';
+      fields(cdata, nc);
+    }
+  } else {
+    const char* fmt = binary ? "binary" : "ascii";
+    f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"PolyData\" version=\"1.0\" byte_order=\"LittleEndian\" "
+         "header_type=\"UInt64\">\n<PolyData>\n<Piece NumberOfPoints=\""
+      << nv << "\" NumberOfVerts=\"0\" NumberOfLines=\"0\" NumberOfStrips=\"0\" NumberOfPolys=\"" << nc
+      << "\">\n";
+    auto arrays = [&](const char* tag, const std::vector<Array>& as, int64_t n) {
+      f << '<' << tag << ">\n";
+      for (const Array& a : as) {
+        f << "<DataArray type=\"Float64\" Name=\"" << xml_name(a.name) << "\" NumberOfComponents=\"" << a.ncomp
+          << "\" format=\"" << fmt << "\">\n";
+        o.doubles(a.data, n * a.ncomp, a.ncomp == 1 ? 9 : a.ncomp);
+        f << "</DataArray>\n";
+      }
+      f << "</" << tag << ">\n";
+    };
+    arrays("PointData", pdata, nv);
+    arrays("CellData", cdata, nc);
+    f << "<Points>\n<DataArray type=\"Float64\" Name=\"Points\" NumberOfComponents=\"3\" format=\"" << fmt
+      << "\">\n";
+    o.doubles(vx, 3 * nv, 3);
+    f << "</DataArray>\n</Points>\n<Polys>\n";
+    std::vector<long long> conn((size_t)(3 * nc)), offs((size_t)nc);
+    for (int64_t c = 0; c < nc; ++c) {
+      for (int k = 0; k < 3; ++k) conn[(size_t)(3 * c + k)] = m.face_verts[(size_t)(3 * leaves[(size_t)c] + k)];
+      offs[(size_t)c] = 3 * (c + 1);
+    }
+    f << "<DataArray type=\"Int64\" Name=\"connectivity\" format=\"" << fmt << "\">\n";
+    o.values(conn.data(), (int64_t)conn.size(), 9, "%lld");
+    f << "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"" << fmt << "\">\n";
+    o.values(offs.data(), (int64_t)offs.size(), 9, "%lld");
+    f << "</DataArray>\n</Polys>\n</Piece>\n</PolyData>\n</VTKFile>\n";
+  }
+  if (!f) {
+    err = "write failed: " + path;
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
 bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& m, int format, std::string& err,
                    const std::vector<VtkExtra>& extra) {
   if (m.nverts() != s.nv || m.nfaces() != s.nf) {
     err = "mesh sizes differ from the state's";
     return false;
   }
-  if (format < kVtkLegacyAscii || format > kVtkXmlBinary) {
-    err = "unknown VTK format";
-    return false;
-  }
-  const bool xml = format >= kVtkXmlAscii, binary = format == kVtkLegacyBinary || format == kVtkXmlBinary;
-  // leaf faces (no kids, src/LpmVtkIO.cpp:23-30): connectivity and cell data
-  std::vector<int64_t> leaves;
-  for (int64_t f = 0; f < s.nf; ++f)
-    if (m.face_kids[(size_t)(4 * f)] < 0) leaves.push_back(f);
+  const bool xml = format >= kVtkXmlAscii;
+  const std::vector<int64_t> leaves = leaf_faces(m);
   const int64_t nc = (int64_t)leaves.size();
   std::vector<double> c_area(nc), c_zeta(nc), c_absv(nc), c_psi(nc), c_u(3 * nc);
   for (int64_t c = 0; c < nc; ++c) {
@@ -302,72 +390,15 @@ bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& 
       for (int k = 0; k < x.ncomp; ++k) rows[(size_t)(x.ncomp * c + k)] = x.data[x.ncomp * leaves[(size_t)c] + k];
     cdata.push_back({x.name.c_str(), x.ncomp, rows.data()});
   }
-  std::ofstream f(path, std::ios::binary | std::ios::trunc);
-  if (!f) {
-    err = "cannot open " + path + " for writing";
-    return false;
-  }
-  VtkOut o{f, binary, xml};
-  if (!xml) {
-    f << "# vtk DataFile Version 3.0\nLPM BVE sphere\n" << (binary ? "BINARY" : "ASCII")
-      << "\nDATASET POLYDATA\nPOINTS " << s.nv << " double\n";
-    o.doubles(s.vx.data(), 3 * s.nv, 3);
-    f << "POLYGONS " << nc << ' ' << 4 * nc << '\n';
-    std::vector<int32_t> conn((size_t)(4 * nc));
-    for (int64_t c = 0; c < nc; ++c) {
-      conn[(size_t)(4 * c)] = 3;
-      for (int k = 0; k < 3; ++k) conn[(size_t)(4 * c + 1 + k)] = m.face_verts[(size_t)(3 * leaves[(size_t)c] + k)];
-    }
-    o.values(conn.data(), (int64_t)conn.size(), 4, "%d");
-    auto fields = [&](const std::vector<Array>& as, int64_t n) {
-      f << "FIELD FieldData " << as.size() << '\n';
-      for (const Array& a : as) {
-        f << legacy_name(a.name) << ' ' << a.ncomp << ' ' << n << " double\n";
-        o.doubles(a.data, n * a.ncomp, a.ncomp == 1 ? 9 : a.ncomp);
-      }
-    };
-    f << "POINT_DATA " << s.nv << '\n';
-    fields(pdata, s.nv);
-    f << "CELL_DATA " << nc << '\n';
-    fields(cdata, nc);
-  } else {
-    const char* fmt = binary ? "binary" : "ascii";
-    f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"PolyData\" version=\"1.0\" byte_order=\"LittleEndian\" "
-         "header_type=\"UInt64\">\n<PolyData>\n<Piece NumberOfPoints=\""
-      << s.nv << "\" NumberOfVerts=\"0\" NumberOfLines=\"0\" NumberOfStrips=\"0\" NumberOfPolys=\"" << nc
-      << "\">\n";
-    auto arrays = [&](const char* tag, const std::vector<Array>& as, int64_t n) {
-      f << '<' << tag << ">\n";
-      for (const Array& a : as) {
-        f << "<DataArray type=\"Float64\" Name=\"" << xml_name(a.name) << "\" NumberOfComponents=\"" << a.ncomp
-          << "\" format=\"" << fmt << "\">\n";
-        o.doubles(a.data, n * a.ncomp, a.ncomp == 1 ? 9 : a.ncomp);
-        f << "</DataArray>\n";
-      }
-      f << "</" << tag << ">\n";
-    };
-    arrays("PointData", pdata, s.nv);
-    arrays("CellData", cdata, nc);
-    f << "<Points>\n<DataArray type=\"Float64\" Name=\"Points\" NumberOfComponents=\"3\" format=\"" << fmt
-      << "\">\n";
-    o.doubles(s.vx.data(), 3 * s.nv, 3);
-    f << "</DataArray>\n</Points>\n<Polys>\n";
-    std::vector<long long> conn((size_t)(3 * nc)), offs((size_t)nc);
-    for (int64_t c = 0; c < nc; ++c) {
-      for (int k = 0; k < 3; ++k) conn[(size_t)(3 * c + k)] = m.face_verts[(size_t)(3 * leaves[(size_t)c] + k)];
-      offs[(size_t)c] = 3 * (c + 1);
-    }
-    f << "<DataArray type=\"Int64\" Name=\"connectivity\" format=\"" << fmt << "\">\n";
-    o.values(conn.data(), (int64_t)conn.size(), 9, "%lld");
-    f << "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"" << fmt << "\">\n";
-    o.values(offs.data(), (int64_t)offs.size(), 9, "%lld");
-    f << "</DataArray>\n</Polys>\n</Piece>\n</PolyData>\n</VTKFile>\n";
-  }
-  if (!f) {
-    err = "write failed: " + path;
-    return false;
-  }
-  return true;
+  return write_polydata(path, format, s.nv, s.vx.data(), m, leaves, pdata, cdata, "LPM BVE sphere", err);
+}
+
+bool write_mesh_vtk(const std::string& path, const TreeMesh& m, int format, std::string& err) {
+  const std::vector<int64_t> leaves = leaf_faces(m);
+  std::vector<double> area(leaves.size());
+  for (size_t c = 0; c < leaves.size(); ++c) area[c] = m.face_area[(size_t)leaves[c]];
+  return write_polydata(path, format, m.nverts(), m.vert_phys.data()->v, m, leaves, {}, {{"area", 1, area.data()}},
+                        "LPM PolyMesh2d", err);
 }
 
 }  // namespace lpm

@@ -71,4 +71,8 @@ struct VtkExtra {
 bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& mesh, int format,
                    std::string& err, const std::vector<VtkExtra>& extra = {});
 
+// PolyMesh2d::outputVtk (src/LpmPolyMesh2d.cpp:79-85): the mesh's leaf faces
+// with their areas as cell data ("area").
+bool write_mesh_vtk(const std::string& path, const TreeMesh& mesh, int format, std::string& err);
+
 }  // namespace lpm

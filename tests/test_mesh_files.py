@@ -193,3 +193,22 @@ def test_truncated_files_fail_cleanly(files, tmp_path):
         p.write_bytes(data[:cut])
         with pytest.raises(L.LpmError):
             L.TreeMesh.from_nc(str(p))
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_mesh_vtk(tmp_path, fmt):
+    """PolyMesh2d::outputVtk: vertices, leaf faces and their areas (parsed here
+    with the GPU tests' independent VTK readers)."""
+    from test_gpu_files import read_legacy, read_xml
+    m = L.TreeMesh(3)
+    path = str(tmp_path / ("m.vtp" if fmt >= 2 else "m.vtk"))
+    L.write_mesh_vtk(3, path, fmt)
+    d = read_xml(path) if fmt >= 2 else read_legacy(path)
+    leaf = m.fmask == 0
+    assert np.array_equal(d["points"], m.vx)
+    polys = d["polys"] if fmt >= 2 else d["polys"][:, 1:]
+    assert np.array_equal(polys, m.fverts[leaf])
+    assert list(d["point"]) == [] and list(d["cell"]) == ["area"]
+    assert np.array_equal(d["cell"]["area"].ravel(), m.farea[leaf])
+    with pytest.raises(L.LpmError):
+        L.write_mesh_vtk(3, path, 7)
