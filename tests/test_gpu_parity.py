@@ -96,9 +96,9 @@ def test_tree_faces_drop_in(omesh):
     c.close()
 
 
-def test_c0_rh_wave_100_steps_ref(omesh):
-    """Config 0: level 4, RH n=4 w=K=0.1, Omega=2 pi, dt=0.01, 100 steps of the
-    reference scheme in FP64: velocity <= 1e-11, positions <= 1e-9 (normwise)."""
+def test_level4_rh_wave_100_steps_ref(omesh):
+    """Level-4 Rossby-Haurwitz wave (n=4, w=K=0.1), Omega=2 pi, dt=0.01, 100 steps
+    of the reference scheme in FP64: velocity <= 1e-11, positions <= 1e-9 (normwise)."""
     m = omesh(4)
     lm = LeafMesh(m)
     vz, fz = rh_field(lm.vx), rh_field(lm.fx)
@@ -227,8 +227,9 @@ def test_launch_shapes_agree(omesh, R, splits):
     c.close()
 
 
-def test_c1_level7_subset(omesh):
-    """Config 1: level 7, 32 Gaussian vortices (sigma 0.1, seed 1, mean removed):
+def test_level7_gaussian_vortices_subset(omesh):
+    """Level 7 (163,842 vertices, 327,680 faces), 32 Gaussian vortices (sigma 0.1,
+    seed 1, mean removed):
     FP64 reference-scheme velocity against the oracle on every 997th target,
     within the 1e-11 velocity bound."""
     from oracle.oracle import Mesh
