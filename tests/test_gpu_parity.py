@@ -113,7 +113,7 @@ def test_level4_rh_wave_100_steps_ref(omesh):
     ev = max(normwise(s["vu"], o.vvel), normwise(s["fu"], o.fvel))
     ex = max(normwise(s["vx"], o.vx), normwise(s["fx"], o.fx))
     ez = max(normwise(s["vzeta"], o.vzeta), normwise(s["fzeta"], o.fzeta))
-    print(f"C0 100 steps: velocity {ev:.3e} positions {ex:.3e} vorticity {ez:.3e}")
+    print(f"level 4, 100 steps: velocity {ev:.3e} positions {ex:.3e} vorticity {ez:.3e}")
     assert ev <= 1e-11 and ex <= 1e-9 and ez <= 1e-9
     c.close()
 
@@ -246,7 +246,7 @@ def test_level7_gaussian_vortices_subset(omesh):
     # the literal 1 - x.y keeps ~5 digits for nearest pairs at level 7; FMA vs
     # separately rounded dot products differ there by ~1e-11 relative
     ev, ef = normwise(s["vu"][::stride], ov), normwise(s["fu"][::stride], of)
-    print(f"C1 level 7 subset: vertex {ev:.3e} face {ef:.3e}")
+    print(f"level 7 subset: vertex {ev:.3e} face {ef:.3e}")
     assert ev < 1e-11 and ef < 1e-11
     c.close()
 
