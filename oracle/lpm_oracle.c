@@ -501,9 +501,10 @@ ORC_EXPORT void orc_streamfn(double* psi, const double* tx, int nt, const double
  * unit sphere and has no cancellation.  Numerator cross(x, y).
  */
 static void velocity_chord(double* u, const double* tx, int nt, const double* fx, const double* fzeta,
-                           const double* farea, const unsigned char* fmask, int nf, int collocated) {
+                           const double* farea, const unsigned char* fmask, int nf, int collocated, int stride) {
+  if (stride < 1) stride = 1;
 #pragma omp parallel for schedule(dynamic, 16)
-  for (int i = 0; i < nt; ++i) {
+  for (int i = 0; i < nt; i += stride) {
     double acc[3] = {0.0, 0.0, 0.0};
     const double* x = &tx[3 * i];
     for (int j = 0; j < nf; ++j) {
@@ -541,8 +542,9 @@ static void streamfn_chord(double* psi, const double* tx, int nt, const double* 
 }
 
 ORC_EXPORT void orc_velocity_chord(double* u, const double* tx, int nt, const double* fx, const double* fzeta,
-                                   const double* farea, const unsigned char* fmask, int nf, int collocated) {
-  velocity_chord(u, tx, nt, fx, fzeta, farea, fmask, nf, collocated);
+                                   const double* farea, const unsigned char* fmask, int nf, int collocated,
+                                   int stride) {
+  velocity_chord(u, tx, nt, fx, fzeta, farea, fmask, nf, collocated, stride);
 }
 ORC_EXPORT void orc_streamfn_chord(double* psi, const double* tx, int nt, const double* fx, const double* fzeta,
                                    const double* farea, const unsigned char* fmask, int nf, int collocated) {
@@ -613,8 +615,8 @@ static void vel_all(int scheme, double* vu, const double* vx, int nv, double* fu
     orc_velocity(vu, vx, nv, fx, fz, fa, fm, nf, 0, 1);
     orc_velocity(fu, fx, nf, fx, fz, fa, fm, nf, 1, 1);
   } else {
-    velocity_chord(vu, vx, nv, fx, fz, fa, fm, nf, 0);
-    velocity_chord(fu, fx, nf, fx, fz, fa, fm, nf, 1);
+    velocity_chord(vu, vx, nv, fx, fz, fa, fm, nf, 0, 1);
+    velocity_chord(fu, fx, nf, fx, fz, fa, fm, nf, 1, 1);
   }
 }
 

@@ -39,7 +39,7 @@ def lib():
         L.orc_icos_tree_init.argtypes = [C.c_int, _dp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _bp, _ip, _ip, _ip]
         L.orc_velocity.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int, C.c_int]
         L.orc_streamfn.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int]
-        L.orc_velocity_chord.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int]
+        L.orc_velocity_chord.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int, C.c_int]
         L.orc_streamfn_chord.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int]
         L.orc_velocity_ld.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int]
         L.orc_rk4_step.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
@@ -137,12 +137,14 @@ def streamfn(tx, fx, fzeta, farea, fmask, collocated: bool):
     return psi
 
 
-def velocity_chord(tx, fx, fzeta, farea, fmask, collocated: bool):
+def velocity_chord(tx, fx, fzeta, farea, fmask, collocated: bool, stride: int = 1):
+    """The same sums with the chord denominator |y - x|^2 / 2 (every `stride`-th
+    target; the others stay zero)."""
     tx = np.ascontiguousarray(tx, np.float64)
     u = np.zeros_like(tx)
     lib().orc_velocity_chord(u, tx, tx.shape[0], np.ascontiguousarray(fx), np.ascontiguousarray(fzeta),
                              np.ascontiguousarray(farea), np.ascontiguousarray(fmask, np.uint8), fx.shape[0],
-                             int(collocated))
+                             int(collocated), int(stride))
     return u
 
 

@@ -251,6 +251,32 @@ def test_level7_gaussian_vortices_subset(omesh):
     c.close()
 
 
+def test_level8_lagrangian_subset():
+    """Level 8 (655,362 vertices, 1,310,720 faces; the benchmark's workload), 32
+    Gaussian vortices (sigma 0.1, seed 1, mean removed): the FP64 Lagrangian-scheme
+    velocity against the oracle's chord-form sums on every 4999th target (no
+    cancellation in |y - x|^2 / 2, so to 1e-12), and the FP32C step kernel against
+    the FP64 sums at every target (1e-6)."""
+    from oracle.oracle import Mesh
+    m = Mesh(8)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=1, nvort=32, sigma=0.1)
+    fz = fz[m.fmask == 0]
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR)
+    a = c.get_state()
+    stride = 4999
+    ov = O.velocity_chord(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False, stride=stride)[::stride]
+    of = O.velocity_chord(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True, stride=stride)[::stride]
+    ev, ef = normwise(a["vu"][::stride], ov), normwise(a["fu"][::stride], of)
+    c.solve(L.PREC_FP32C, L.SCHEME_LAGR)
+    b = c.get_state()
+    e32 = max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"]))
+    print(f"level 8 subset: FP64 vertex {ev:.3e} face {ef:.3e}; FP32C vs FP64 {e32:.3e}")
+    assert ev < 1e-12 and ef < 1e-12 and e32 < 1e-6
+    c.close()
+
+
 def test_solid_body_rotation_gpu(omesh):
     """tests/LpmBVETest.cpp: zeta = 2 Omega z moves particles rigidly.  The
     renormalised scheme tracks the exact rotation at level 4; the reference
