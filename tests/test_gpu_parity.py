@@ -255,8 +255,9 @@ def test_level8_lagrangian_subset():
     """Level 8 (655,362 vertices, 1,310,720 faces; the benchmark's workload), 32
     Gaussian vortices (sigma 0.1, seed 1, mean removed): the FP64 Lagrangian-scheme
     velocity against the oracle's chord-form sums on every 4999th target (no
-    cancellation in |y - x|^2 / 2, so to 1e-12), and the FP32C step kernel against
-    the FP64 sums at every target (1e-6)."""
+    cancellation in |y - x|^2 / 2, so to 1e-12), the FP32C step kernel against
+    the FP64 sums at every target (1e-6), and the reference scheme's FP64 sums
+    (literal 1 - x.y) against the oracle's on the same subset (1e-11)."""
     from oracle.oracle import Mesh
     m = Mesh(8)
     lm = LeafMesh(m)
@@ -272,8 +273,15 @@ def test_level8_lagrangian_subset():
     c.solve(L.PREC_FP32C, L.SCHEME_LAGR)
     b = c.get_state()
     e32 = max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"]))
-    print(f"level 8 subset: FP64 vertex {ev:.3e} face {ef:.3e}; FP32C vs FP64 {e32:.3e}")
-    assert ev < 1e-12 and ef < 1e-12 and e32 < 1e-6
+    # the reference scheme's literal 1 - x.y, within the 1e-11 velocity bound
+    c.solve(L.PREC_FP64, L.SCHEME_REF)
+    r = c.get_state()
+    rv = O.velocity(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False, stride=stride)[::stride]
+    rf = O.velocity(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True, stride=stride)[::stride]
+    erv, erf = normwise(r["vu"][::stride], rv), normwise(r["fu"][::stride], rf)
+    print(f"level 8 subset: FP64 chord vertex {ev:.3e} face {ef:.3e}; FP32C vs FP64 {e32:.3e}; "
+          f"FP64 dot vertex {erv:.3e} face {erf:.3e}")
+    assert ev < 1e-12 and ef < 1e-12 and e32 < 1e-6 and erv < 1e-11 and erf < 1e-11
     c.close()
 
 
