@@ -242,7 +242,10 @@ int lpm_bve_checkpoint(lpm_bve_ctx* ctx, const char* path, int level, double tim
 /* Resume from a checkpoint: the state, its velocity (with the precision and
  * form that produced it), psi and absolute vorticity, so that the next steps
  * are those the checkpointed context would have taken.  time and omega
- * (non-NULL) receive the stored values. */
+ * (non-NULL) receive the stored values.  No counterpart in the reference,
+ * whose restart (PolyMesh2d(const PolyMeshReader&), src/LpmPolyMesh2d.cpp:20-28)
+ * restores the mesh only; for callers that stop and resume runs (the
+ * serving loop's drain and resume). */
 int lpm_bve_restore(lpm_bve_ctx* ctx, const char* path, double* time, double* omega);
 
 /* VTK polydata of the state on the leaf faces of its tree (`level` as for
