@@ -57,6 +57,9 @@ typedef enum {
   LPM_VTK_XML_BINARY = 3     /* .vtp, inline base64 */
 } lpm_vtk_format;
 
+/* Status text, version and device count: no counterpart in the reference
+ * (whose errors are C++ exceptions, LPM_THROW_IF in src/LpmDefs.hpp); for
+ * callers across the C ABI. */
 const char* lpm_last_error(void);
 int lpm_version(void); /* major*10000 + minor*100 + patch */
 int lpm_device_count(int* count);
@@ -112,11 +115,15 @@ int lpm_mesh_read_nc(const char* path, double* vx, double* vlag, double* fx, dou
  * amp_scale times one normal (amplitude). centers [nvort][3], amps [nvort]. */
 int lpm_ic_gauss_params(uint64_t seed, int nvort, double amp_scale, double* centers, double* amps);
 
+/* VorticityInitialCondition::eval (src/LpmVorticityGallery.hpp:13-40) of the
+ * composite field above at n points xyz [n][3] -> zeta [n]. */
 int lpm_ic_eval(const double* xyz, int64_t n, double* zeta, double sbr, int rh_n, double rh_w, double rh_K,
                 int nvort, const double* centers, const double* amps, double sigma);
 
 /* The same generator's raw draws: nuniform values in [0,1), then nnormal
- * standard normals (used to draw ensemble and request parameters). */
+ * standard normals (used to draw ensemble and request parameters).  No
+ * counterpart in the reference; for the seeded batching and serving
+ * workloads. */
 int lpm_rng_draws(uint64_t seed, int64_t nuniform, double* uniform, int64_t nnormal, double* normal);
 
 /* The Poisson problem of SpherePoisson (src/LpmSpherePoisson.hpp:60-80):
@@ -125,7 +132,9 @@ int lpm_rng_draws(uint64_t seed, int64_t nuniform, double* uniform, int64_t nnor
  * f [n] and u [n][3] may each be NULL. */
 int lpm_poisson_y54(const double* xyz, int64_t n, double* f, double* u);
 
-/* Area-weighted mean of zeta over unmasked faces, summed in index order. */
+/* Area-weighted mean of zeta over unmasked faces, summed in index order (the
+ * mean that the Gaussian-vortex initial fields subtract; no separate function
+ * in the reference). */
 int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, double* mean);
 
 /* ------------------------------------------------------------------ */
@@ -165,6 +174,10 @@ int lpm_bve_face_solve(double* psi, double* u, const double* fx, const double* f
 /* ------------------------------------------------------------------ */
 typedef struct lpm_bve_ctx lpm_bve_ctx;
 
+/* The BVESphere(nmaxverts, nmaxedges, nmaxfaces) and BVERK4(dt, Omega)
+ * constructors (src/LpmBVESphere.cpp:10-49, src/LpmBVERK4.cpp:5-37) and their
+ * destructors: a context with its own stream on `device`; sizes come with the
+ * state. */
 int lpm_bve_create(int device, lpm_bve_ctx** ctx);
 /* A context that queues its work on a caller's stream (a hipStream_t of
  * `device`, not owned: it must outlive the context).  Contexts sharing a
@@ -174,9 +187,12 @@ int lpm_bve_create(int device, lpm_bve_ctx** ctx);
 int lpm_bve_create_on_stream(int device, void* stream, lpm_bve_ctx** ctx);
 int lpm_bve_destroy(lpm_bve_ctx* ctx);
 
-/* Upload a particle state (host arrays; copied).  vx [nv][3], vzeta [nv];
- * fx [nf][3], fzeta, farea, fmask [nf] over tree faces.  Velocities become
- * invalid until lpm_bve_solve, lpm_bve_set_velocity or the first step. */
+/* Upload a particle state (host arrays; copied): the physVerts, relVortVerts,
+ * physFaces, relVortFaces, faces.area and faces.mask views that treeInit and
+ * init_vorticity fill (src/LpmBVESphere.cpp:126-149) and advance_timestep
+ * reads (src/LpmBVERK4.hpp:29-34).  vx [nv][3], vzeta [nv]; fx [nf][3],
+ * fzeta, farea, fmask [nf] over tree faces.  Velocities become invalid until
+ * lpm_bve_solve, lpm_bve_set_velocity or the first step. */
 int lpm_bve_set_state(lpm_bve_ctx* ctx, int64_t nv, const double* vx, const double* vzeta, int64_t nf,
                       const double* fx, const double* fzeta, const double* farea, const uint8_t* fmask);
 
@@ -205,11 +221,14 @@ typedef struct lpm_ic_params {
 int lpm_bve_set_state_level(lpm_bve_ctx* ctx, int level, const lpm_ic_params* ic);
 
 /* Free the cached meshes of `device` (all devices if device < 0) that no
- * context uses; *kept (may be NULL) receives the number still in use. */
+ * context uses; *kept (may be NULL) receives the number still in use.  No
+ * counterpart in the reference (it rebuilds the mesh per object). */
 int lpm_mesh_cache_clear(int device, int* kept);
 
-/* Provide the entry velocity (the BVERK4 precondition, F1: velocity valid at
- * the current state).  vu [nv][3], fu [nf][3]. */
+/* Provide the entry velocity: the velocityVerts / velocityFaces views that
+ * advance_timestep requires to hold the velocity at the current state
+ * (src/LpmBVERK4.hpp:32-33, stage 1 at src/LpmBVERK4_Impl.hpp:61-68).
+ * vu [nv][3], fu [nf][3]. */
 int lpm_bve_set_velocity(lpm_bve_ctx* ctx, const double* vu, const double* fu);
 
 /* Velocity (and, if want_psi, stream function) at the current state, as the
@@ -223,7 +242,8 @@ int lpm_bve_solve(lpm_bve_ctx* ctx, int prec, int scheme, int want_psi);
  * captured hipGraph when use_graph != 0. */
 int lpm_bve_advance(lpm_bve_ctx* ctx, double dt, double omega, int scheme, int prec, int nsteps, int use_graph);
 
-/* Copy the resident state to host arrays (any pointer may be NULL).
+/* Copy the resident state to host arrays (any pointer may be NULL), as
+ * BVESphere::updateHost does for its host mirrors (src/LpmBVESphere.cpp:77-88).
  * Blocks until the context's stream has drained. */
 int lpm_bve_get_state(lpm_bve_ctx* ctx, double* vx, double* vzeta, double* vu, double* vpsi, double* fx,
                       double* fzeta, double* fu, double* fpsi);
@@ -271,10 +291,15 @@ typedef struct {
 int lpm_bve_write_vtk_arrays(lpm_bve_ctx* ctx, const char* path, int level, double omega, int format,
                              const lpm_vtk_array* arrays, int narrays);
 
-/* Total circulation sum over unmasked faces of zeta * area (FP64, device). */
+/* Total circulation sum over unmasked faces of zeta * area (FP64, device):
+ * the conservation check of the Lagrangian scheme; the reference computes it
+ * in its drivers from the host mirrors. */
 int lpm_bve_circulation(lpm_bve_ctx* ctx, double* circ);
 
-/* Block until all work queued on the context has finished. */
+/* Context control, launch shape and timing: no counterpart in the reference
+ * (Kokkos fences and chooses team sizes itself); for asynchronous callers,
+ * the serving loop and measurement.  Block until all work queued on the
+ * context has finished. */
 int lpm_bve_synchronize(lpm_bve_ctx* ctx);
 /* Non-blocking: *done = 1 when the context's asynchronous work (solves, steps,
  * ensemble steps) queued so far has finished. */
@@ -313,6 +338,7 @@ int lpm_bve_enable_timing(lpm_bve_ctx* ctx, int on);
 int lpm_bve_state_bytes(int64_t nv, int64_t nactive, int64_t* bytes);
 
 /* ------------------------------------------------------------------ */
+/* No counterpart in the reference (the batching form's autotuner).    */
 /* Launch shapes keyed on the number of targets N (vertices + active   */
 /* faces).  A context whose launch is automatic (lpm_bve_set_launch    */
 /* with targets_per_lane 0, the default) uses the rule for its N, so a */
@@ -328,6 +354,7 @@ int lpm_bve_autotune(lpm_bve_ctx* ctx, int prec, int scheme, int reps, int* best
                      int* best_source_splits, double* best_ms);
 
 /* ------------------------------------------------------------------ */
+/* No counterpart in the reference (the batching form).                */
 /* Ensemble: independent problems (contexts, possibly of different     */
 /* mesh levels) advanced together.  Each of the four evaluations of a  */
 /* step runs as ONE persistent pair kernel over the flattened list of  */
