@@ -28,6 +28,8 @@ Reported: completion latency (finish - arrival) p50 / p99 and aggregate
 pair interactions per second over the run.
 
     python -m lpmkokkos_amd.serve --requests 2000 --budget-gb 200
+    python -m lpmkokkos_amd.serve --requests 200 --stop-after-steps 500 --checkpoint-dir ck
+    python -m lpmkokkos_amd.serve --resume ck
 """
 from __future__ import annotations
 
@@ -430,7 +432,14 @@ def main(argv=None) -> int:
     ap.add_argument("--max-level", type=int, default=8)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--stop-after-steps", type=int, default=None,
+                    help="drain after this many batched steps into --checkpoint-dir")
+    ap.add_argument("--checkpoint-dir", default=None)
+    ap.add_argument("--resume", default=None, metavar="DIR",
+                    help="serve the requests a drained server left in DIR (its manifest.json)")
     args = ap.parse_args(argv)
+    if args.stop_after_steps is not None and not args.checkpoint_dir:
+        ap.error("--stop-after-steps needs --checkpoint-dir")
     # one hardware queue per lane stream (read at the first HIP call, below;
     # the runtime's default is 4)
     if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
@@ -439,13 +448,24 @@ def main(argv=None) -> int:
     tot = sum(probs.values())
     probs = {l: p / tot for l, p in probs.items()}
     log = lambda s: print(s, file=sys.stderr, flush=True)
+    if args.resume:
+        reqs = load_manifest(args.resume)
+        with open(os.path.join(args.resume, "manifest.json")) as f:
+            saved = json.load(f)
+        srv = Server(budget_bytes=args.budget_gb * 1e9, log=log, dt=saved["dt"], omega=saved["omega"],
+                     prec=saved["prec"])
+        res = srv.run(reqs, stop_after_steps=args.stop_after_steps, checkpoint_dir=args.checkpoint_dir)
+        srv.close()
+        res.update({"resumed_from": args.resume, "budget_gb": args.budget_gb})
+        print(json.dumps(res))
+        return 0
     svc = measure_service_times(sorted(probs), steps=args.steps)
     log(f"[serve] service seconds per request by level: {svc}")
     mean_service = sum(probs[l] * svc[l] for l in probs)
     rate = args.load / mean_service
     reqs = make_requests(args.requests, rate, seed=args.seed, level_probs=probs, steps=args.steps)
     srv = Server(budget_bytes=args.budget_gb * 1e9, log=log)
-    res = srv.run(reqs)
+    res = srv.run(reqs, stop_after_steps=args.stop_after_steps, checkpoint_dir=args.checkpoint_dir)
     srv.close()
     res.update({"arrival_rate_per_s": rate, "service_s_by_level": svc, "load": args.load,
                 "level_probs": probs, "budget_gb": args.budget_gb})
