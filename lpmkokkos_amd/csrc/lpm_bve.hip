@@ -40,6 +40,28 @@ __device__ __forceinline__ double rcp64(double a) {
   return fma(r, fma(e, e, e), r);
 }
 
+// 1/a, 1/b, 1/c, 1/d of four positive doubles from one reciprocal:
+// p = 1/(ab cd) from the v_rcp_f64 estimate r refined to second order
+// (e = 1 - P r, p = r (1 + e + e^2), as in rcp64), then 1/a = b (cd p),
+// 1/b = a (cd p), 1/c = d (ab p), 1/d = c (ab p).  12 FP64 operations and one
+// v_rcp_f64 replace four rcp64 calls (12 FP64 operations, 8 conversions and
+// 4 v_rcp_f32); each result carries about five roundings.  The product must
+// lie in the FP64 normal range: four pair denominators of distinct points
+// (|y - x|^2 >= 1e-70 each) or padding records do.  A zero denominator
+// makes all four results non-finite.
+__device__ __forceinline__ void rcp4_64(double a, double b, double c, double d, double& ia, double& ib,
+                                        double& ic, double& id) {
+  const double ab = a * b, cd = c * d, P = ab * cd;
+  const double r = __builtin_amdgcn_rcp(P);
+  const double e = fma(-P, r, 1.0);
+  const double p = fma(r, fma(e, e, e), r);
+  const double iab = cd * p, icd = ab * p;
+  ia = b * iab;
+  ib = a * iab;
+  ic = d * icd;
+  id = c * icd;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
@@ -326,6 +348,66 @@ __device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, b
   if (PSI) S[3] = fma(q, log(den), S[3]);
 }
 
+// dot_term's arithmetic for one target and four sources with one shared
+// reciprocal (rcp4_64).  The four pairs belong to one target, so a degenerate
+// pair (a target on a source that is not its own, as a divided face's centre
+// on its middle child's) leaves every other target's sums as they are.
+__device__ __forceinline__ double dot_den(const double (&x)[3], const Src64& s) {
+  return 1.0 - (x[0] * s.a.x + x[1] * s.a.y + x[2] * s.a.z);
+}
+
+template <bool PSI>
+__device__ __forceinline__ void dot_acc(double inv, double den, const Src64& s, double (&S)[kNC]) {
+  S[0] = fma(inv, s.b.x, S[0]);
+  S[1] = fma(inv, s.b.y, S[1]);
+  S[2] = fma(inv, s.b.z, S[2]);
+  if (PSI) S[3] = fma(s.a.w, log(den), S[3]);
+}
+
+template <bool PSI>
+__device__ __forceinline__ void dot_terms4(const double (&x)[3], const Src64& s0, const Src64& s1,
+                                           const Src64& s2, const Src64& s3, double (&S)[kNC]) {
+  const double a = dot_den(x, s0), b = dot_den(x, s1), c = dot_den(x, s2), d = dot_den(x, s3);
+  double ia, ib, ic, id;
+  rcp4_64(a, b, c, d, ia, ib, ic, id);
+  dot_acc<PSI>(ia, a, s0, S);
+  dot_acc<PSI>(ib, b, s1, S);
+  dot_acc<PSI>(ic, c, s2, S);
+  dot_acc<PSI>(id, d, s3, S);
+}
+
+// Chord far form (tile-local records, see pair_block_f64) for one target and
+// four sources with one shared reciprocal: r^2 = |D|^2 + |dx|^2 + 2 D.dx,
+// F += q {1, dx}/r^2, and with PSI F[4] += q ln(r^2 / 2).
+__device__ __forceinline__ double chord_r2(const double (&D2)[3], double E, const Src64& s) {
+  double r2 = E + s.a.w;
+  r2 = fma(D2[0], s.a.x, r2);
+  r2 = fma(D2[1], s.a.y, r2);
+  return fma(D2[2], s.a.z, r2);
+}
+
+template <bool PSI>
+__device__ __forceinline__ void chord_acc(double inv, double r2, const Src64& s, double (&F)[kNC + 1]) {
+  F[0] = fma(inv, s.b.x, F[0]);
+  F[1] = fma(inv, s.b.y, F[1]);
+  F[2] = fma(inv, s.b.z, F[2]);
+  F[3] = fma(inv, s.b.w, F[3]);
+  if (PSI) F[4] = fma(s.b.x, log(0.5 * r2), F[4]);
+}
+
+template <bool PSI>
+__device__ __forceinline__ void chord_terms4(const double (&D2)[3], double E, const Src64& s0, const Src64& s1,
+                                             const Src64& s2, const Src64& s3, double (&F)[kNC + 1]) {
+  const double a = chord_r2(D2, E, s0), b = chord_r2(D2, E, s1);
+  const double c = chord_r2(D2, E, s2), d = chord_r2(D2, E, s3);
+  double ia, ib, ic, id;
+  rcp4_64(a, b, c, d, ia, ib, ic, id);
+  chord_acc<PSI>(ia, a, s0, F);
+  chord_acc<PSI>(ib, b, s1, F);
+  chord_acc<PSI>(ic, c, s2, F);
+  chord_acc<PSI>(id, d, s3, F);
+}
+
 // FP64 pair sums of one target block (R targets per lane) over one source
 // split.  LDS: source tiles are staged through LDS (double-buffered, one
 // coalesced 16 KB copy per tile, tile t+1 loaded while tile t is summed);
@@ -366,11 +448,10 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
     const bool has_self = wave_has_self<R>(self, base);
     if constexpr (FORM == FORM_DOT) {
       if (!has_self) {
-#pragma unroll 4
-        for (int j = 0; j < kTile; ++j) {
-          const Src64 sj = ts[j];
+        for (int j = 0; j < kTile; j += 4) {
+          const Src64 s0 = ts[j], s1 = ts[j + 1], s2 = ts[j + 2], s3 = ts[j + 3];
 #pragma unroll
-          for (int r = 0; r < R; ++r) dot_term<PSI, false>(x[r], sj, false, S[r]);
+          for (int r = 0; r < R; ++r) dot_terms4<PSI>(x[r], s0, s1, s2, s3, S[r]);
         }
       } else {
         for (int j = 0; j < kTile; ++j) {
@@ -415,22 +496,10 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
           E[r] = fma(D[r][0], D[r][0], fma(D[r][1], D[r][1], D[r][2] * D[r][2]));
           for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0;
         }
-#pragma unroll 2
-        for (int j = 0; j < kTile; ++j) {
-          const Src64 sj = ts[j];
+        for (int j = 0; j < kTile; j += 4) {
+          const Src64 s0 = ts[j], s1 = ts[j + 1], s2 = ts[j + 2], s3 = ts[j + 3];
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            double r2 = E[r] + sj.a.w;
-            r2 = fma(D2[r][0], sj.a.x, r2);
-            r2 = fma(D2[r][1], sj.a.y, r2);
-            r2 = fma(D2[r][2], sj.a.z, r2);
-            const double inv = rcp64(r2);
-            F[r][0] = fma(inv, sj.b.x, F[r][0]);
-            F[r][1] = fma(inv, sj.b.y, F[r][1]);
-            F[r][2] = fma(inv, sj.b.z, F[r][2]);
-            F[r][3] = fma(inv, sj.b.w, F[r][3]);
-            if (PSI) F[r][4] = fma(sj.b.x, log(0.5 * r2), F[r][4]);
-          }
+          for (int r = 0; r < R; ++r) chord_terms4<PSI>(D2[r], E[r], s0, s1, s2, s3, F[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
