@@ -73,8 +73,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // FP64 source record (64 bytes).  Dot form (k_prep_f64_dot):
 // a = {y, q}, b = {q y, 0} with q = (-zeta A)/(4 pi).  Tile-local |y - x|^2/2
-// form (k_prep_f64_tiles): a = {d, |d|^2}, b = {q, q d} with d = y - c the
-// offset from the tile centre c and q = (-zeta A)/(2 pi).
+// form (k_prep_f64_tiles): a = {d, |d|^2}, b = {q y.n, q y.e1, q y.e2, q}
+// with d = y - c the offset from the tile centre c, q = (-zeta A)/(2 pi) and
+// y written in the tile's orthonormal frame (n, e1, e2; tile_frame).
 struct Src64 {
   double4 a, b;
 };
@@ -225,8 +226,9 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, do
   }
 }
 
-// One workgroup per tile: centre (mean of its sources) and FP64 records;
-// padding records sit 8 units from the centre with q = 0.
+// One workgroup per tile: centre (mean of its sources), frame (at
+// ctr[ntiles + 3t + {0,1,2}] = n, e1, e2) and FP64 records; padding records
+// sit 8 units from the centre with q = 0.
 __global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ src, double4* __restrict__ ctr,
                                                           Pos3 pos, const double* __restrict__ zeta,
                                                           const double* __restrict__ area,
@@ -252,19 +254,29 @@ __global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ sr
   }
   __syncthreads();
   const double inv = 1.0 / min(kTile, ns - t * kTile);
-  double c[3];
+  double c[4];
   for (int k = 0; k < 3; ++k) c[k] = ((red[k][0] + red[k][1]) + (red[k][2] + red[k][3])) * inv;
+  c[3] = 0.0;
+  double n[3], e1[3], e2[3];
+  tile_frame(c, n, e1, e2);
   Src64 r;
   if (valid) {
     const double dx = p0 - c[0], dy = p1 - c[1], dz = p2 - c[2];
     r.a = make_double4(dx, dy, dz, fma(dx, dx, fma(dy, dy, dz * dz)));
-    r.b = make_double4(q, q * dx, q * dy, q * dz);
+    r.b = make_double4(q * (p0 * n[0] + p1 * n[1] + p2 * n[2]), q * (p0 * e1[0] + p1 * e1[1] + p2 * e1[2]),
+                       q * (p0 * e2[0] + p1 * e2[1] + p2 * e2[2]), q);
   } else {
     r.a = make_double4(8.0, 8.0, 8.0, 192.0);
     r.b = make_double4(0.0, 0.0, 0.0, 0.0);
   }
   src[j] = r;
-  if (threadIdx.x == 0) ctr[t] = make_double4(c[0], c[1], c[2], 0.0);
+  if (threadIdx.x == 0) {
+    ctr[t] = make_double4(c[0], c[1], c[2], 0.0);
+    double4* fr = ctr + gridDim.x + 3 * (long)t;
+    fr[0] = make_double4(n[0], n[1], n[2], 0.0);
+    fr[1] = make_double4(e1[0], e1[1], e1[2], 0.0);
+    fr[2] = make_double4(e2[0], e2[1], e2[2], 0.0);
+  }
 }
 
 // Sum of per-tile q sums in tile order (one workgroup; deterministic).
@@ -378,7 +390,7 @@ __device__ __forceinline__ void dot_terms4(const double (&x)[3], const Src64& s0
 
 // Chord far form (tile-local records, see pair_block_f64) for one target and
 // four sources with one shared reciprocal: r^2 = |D|^2 + |dx|^2 + 2 D.dx,
-// F += q {1, dx}/r^2, and with PSI F[4] += q ln(r^2 / 2).
+// F[0..2] += q y/r^2 in frame components, and with PSI F[3] += q ln(r^2 / 2).
 __device__ __forceinline__ double chord_r2(const double (&D2)[3], double E, const Src64& s) {
   double r2 = E + s.a.w;
   r2 = fma(D2[0], s.a.x, r2);
@@ -387,17 +399,16 @@ __device__ __forceinline__ double chord_r2(const double (&D2)[3], double E, cons
 }
 
 template <bool PSI>
-__device__ __forceinline__ void chord_acc(double inv, double r2, const Src64& s, double (&F)[kNC + 1]) {
+__device__ __forceinline__ void chord_acc(double inv, double r2, const Src64& s, double (&F)[kNC]) {
   F[0] = fma(inv, s.b.x, F[0]);
   F[1] = fma(inv, s.b.y, F[1]);
   F[2] = fma(inv, s.b.z, F[2]);
-  F[3] = fma(inv, s.b.w, F[3]);
-  if (PSI) F[4] = fma(s.b.x, log(0.5 * r2), F[4]);
+  if (PSI) F[3] = fma(s.b.w, log(0.5 * r2), F[3]);
 }
 
 template <bool PSI>
 __device__ __forceinline__ void chord_terms4(const double (&D2)[3], double E, const Src64& s0, const Src64& s1,
-                                             const Src64& s2, const Src64& s3, double (&F)[kNC + 1]) {
+                                             const Src64& s2, const Src64& s3, double (&F)[kNC]) {
   const double a = chord_r2(D2, E, s0), b = chord_r2(D2, E, s1);
   const double c = chord_r2(D2, E, s2), d = chord_r2(D2, E, s3);
   double ia, ib, ic, id;
@@ -416,9 +427,10 @@ __device__ __forceinline__ void chord_terms4(const double (&D2)[3], double E, co
 //   FORM_DOT: dot_term for every pair.
 //   FORM_CHORD (|y - x|^2/2 form, tile-local records): with D = c - x per
 //     target and tile, tiles holding a target's own source form d = D + dx
-//     explicitly and drop the self pair; other tiles form
-//     r^2 = |D|^2 + |dx|^2 + 2 D.dx and sum F = q/r^2, G = q dx/r^2, S += D F + G
-//     (12 FP64 operations per pair instead of 14).
+//     explicitly, drop the self pair and sum S += q d/|d|^2; other tiles form
+//     r^2 = |D|^2 + |dx|^2 + 2 D.dx and sum F = q y/r^2 in the tile frame,
+//     S += n F0 + e1 F1 + e2 F2.  The two differ by the component
+//     -x sum q/r^2, which the finalising cross product with x cancels.
 template <int R, int FORM, bool PSI, bool LDS>
 __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, const double4* __restrict__ ctr,
                                                const PairArgs& a, int bx, int by, Src64 (*sh)[kTile]) {
@@ -476,7 +488,7 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
           for (int r = 0; r < R; ++r) {
             const double e0 = D[r][0] + sj.a.x, e1 = D[r][1] + sj.a.y, e2 = D[r][2] + sj.a.z;
             double r2 = fma(e0, e0, fma(e1, e1, e2 * e2));
-            double q = sj.b.x;
+            double q = sj.b.w;
             if (base + j == self[r]) {
               r2 = 1.0;
               q = 0.0;
@@ -489,24 +501,26 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
           }
         }
       } else {
-        double D2[R][3], E[R], F[R][kNC + 1];
+        double D2[R][3], E[R], F[R][kNC];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           for (int k = 0; k < 3; ++k) D2[r][k] = 2.0 * D[r][k];
           E[r] = fma(D[r][0], D[r][0], fma(D[r][1], D[r][1], D[r][2] * D[r][2]));
-          for (int k = 0; k <= kNC; ++k) F[r][k] = 0.0;
+          for (int k = 0; k < kNC; ++k) F[r][k] = 0.0;
         }
         for (int j = 0; j < kTile; j += 4) {
           const Src64 s0 = ts[j], s1 = ts[j + 1], s2 = ts[j + 2], s3 = ts[j + 3];
 #pragma unroll
           for (int r = 0; r < R; ++r) chord_terms4<PSI>(D2[r], E[r], s0, s1, s2, s3, F[r]);
         }
+        const double4* fr = ctr + a.ntiles + 3 * (long)t;
+        const double4 fn = fr[0], f1 = fr[1], f2 = fr[2];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          S[r][0] += fma(D[r][0], F[r][0], F[r][1]);
-          S[r][1] += fma(D[r][1], F[r][0], F[r][2]);
-          S[r][2] += fma(D[r][2], F[r][0], F[r][3]);
-          S[r][3] += F[r][4];
+          S[r][0] += fma(fn.x, F[r][0], fma(f1.x, F[r][1], f2.x * F[r][2]));
+          S[r][1] += fma(fn.y, F[r][0], fma(f1.y, F[r][1], f2.y * F[r][2]));
+          S[r][2] += fma(fn.z, F[r][0], fma(f1.z, F[r][1], f2.z * F[r][2]));
+          S[r][3] += F[r][3];
         }
       }
     }
