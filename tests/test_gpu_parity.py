@@ -62,6 +62,33 @@ def test_functor_entry_points(omesh):
         d.free()
 
 
+def test_coincident_target_leaves_other_targets_alone(omesh):
+    """A vertex target exactly on a source that is not its own (1 - x.y = 0 exactly, as a
+    divided face's centre on its middle child's) makes only its own row singular, as in the
+    reference's literal division: the FP64 sums share one reciprocal over four pairs of one
+    target, never across targets.  Every other target matches the oracle."""
+    from helpers import DeviceArray
+    m = omesh(3)
+    vx, fx = m.vx.copy(), m.fx.copy()
+    fz = rh_field(fx)
+    j = int(np.flatnonzero(m.fmask == 0)[5])
+    fx[j] = (0.0, 0.0, 1.0)
+    vx[0] = (0.0, 0.0, 1.0)
+    assert 1.0 - float(vx[0] @ fx[j]) == 0.0
+    dv, df, dz, da, dm = (DeviceArray(h) for h in (vx, fx, fz, m.farea, m.fmask))
+    du = DeviceArray.zeros(vx.shape)
+    L._lib.call("lpm_bve_vertex_velocity", L._lib.ptr(du), L._lib.ptr(dv), m.nv, L._lib.ptr(df),
+                L._lib.ptr(dz), L._lib.ptr(da), L._lib.ptr(dm), m.nf, None)
+    u = du.numpy()
+    ov = O.velocity(vx, fx, fz, m.farea, m.fmask, False)
+    assert m.nv > 256  # targets i and i + 256 share a lane at this size
+    assert not np.all(np.isfinite(u[0])) and not np.all(np.isfinite(ov[0]))
+    assert np.all(np.isfinite(u[1:]))
+    assert normwise(u[1:], ov[1:]) < 1e-13
+    for d in (dv, df, dz, da, dm, du):
+        d.free()
+
+
 @pytest.mark.parametrize("level", [0, 2, 4])
 def test_solve_fp64_ref(omesh, level):
     m = omesh(level)
