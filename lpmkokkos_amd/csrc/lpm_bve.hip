@@ -1283,15 +1283,23 @@ Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
   }
   Shape sh;
   // default without a rule: four targets per lane on large problems (half the
-  // LDS reads per pair of two; 1,920 workgroups x 2 source splits at level 8),
-  // two below a million targets, where four would leave too few workgroups
+  // LDS reads per pair of two; 1,921 target blocks x 18 source splits at
+  // 1.97M targets), two below a million targets, where four would leave too
+  // few workgroups
   sh.R = (R_req >= 1 && R_req <= 4) ? R_req : (nt >= (1 << 20) ? 4 : 2);
   sh.ntiles = std::max(1, cdiv(ns, kTile));
   const int wg_x = cdiv(nt, (long)kBlock * sh.R);
   int splits = splits_req;
   if (splits <= 0) {
-    // aim for >= 2048 workgroups (8 per CU on 256 CUs) with >= 4 tiles per split
-    splits = std::max(1, std::min(cdiv(2048, wg_x), std::max(1, sh.ntiles / 4)));
+    // at least 2048 workgroups (8 per CU on 256 CUs) with >= 4 tiles per
+    // split; on large problems, short workgroups: about 32768 of them with
+    // >= 64 tiles each, at most 32 splits.  Workgroup durations vary, so the
+    // last ones to finish leave CUs idle for about one workgroup's duration:
+    // with 1.31M sources and 1.97M targets the FP32C launch took 457 ms with
+    // 2 splits (workgroups of ~116 ms) and 421-424 ms with 16 to 33.
+    const int fill = std::min(cdiv(2048, wg_x), std::max(1, sh.ntiles / 4));
+    const int tail = std::min({cdiv(32768, wg_x), sh.ntiles / 64, 32});
+    splits = std::max({1, fill, tail});
   }
   splits = std::min(splits, sh.ntiles);
   sh.tiles_per_split = cdiv(sh.ntiles, splits);
