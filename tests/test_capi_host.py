@@ -99,6 +99,22 @@ def test_config3_members_build():
     assert E.state_bytes(655362, 1310720) > 29 * 8 * (655362 + 1310720)
 
 
+@pytest.mark.parametrize("level,splits", [(6, 9), (7, 20), (8, 18)])
+def test_state_bytes_count_the_default_split_partials(level, splits):
+    """The memory a context plans (and serving admission charges) holds the
+    default launch's split partials: 4 FP64 sums per target and source split,
+    with 18 splits at level 8 and 20 at level 7 (short workgroups on large
+    problems) and 9 at level 6."""
+    from lpmkokkos_amd import ensemble as E
+    nv, na = 2 + 10 * 4 ** level, 20 * 4 ** level
+    nt = nv + na
+    ntiles = -(-na // 256)
+    ns_pad = 256 * ntiles
+    state = 8 * nt * 30 + 4 * (nt + na)
+    records = ns_pad * (64 + 32 + 8) + (4 * ntiles * 32 + 256) + (4 * ntiles + 256)
+    assert E.state_bytes(nv, na) == state + records + splits * 4 * nt * 8 + 8 * 256
+
+
 def test_request_generator():
     from lpmkokkos_amd import serve as S
     reqs = S.make_requests(2000, rate=0.5, seed=4)
