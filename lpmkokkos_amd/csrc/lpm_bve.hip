@@ -40,15 +40,16 @@ __device__ __forceinline__ double rcp64(double a) {
   return fma(r, fma(e, e, e), r);
 }
 
-// 1/a, 1/b, 1/c, 1/d of four positive doubles from one reciprocal:
+// 1/a, 1/b, 1/c, 1/d of four nonzero doubles from one reciprocal:
 // p = 1/(ab cd) from the v_rcp_f64 estimate r refined to second order
 // (e = 1 - P r, p = r (1 + e + e^2), as in rcp64), then 1/a = b (cd p),
 // 1/b = a (cd p), 1/c = d (ab p), 1/d = c (ab p).  12 FP64 operations and one
 // v_rcp_f64 replace four rcp64 calls (12 FP64 operations, 8 conversions and
-// 4 v_rcp_f32); each result carries about five roundings.  The product must
-// lie in the FP64 normal range: four pair denominators of distinct points
-// (|y - x|^2 >= 1e-70 each) or padding records do.  A zero denominator
-// makes all four results non-finite.
+// 4 v_rcp_f32); each result carries about five roundings.  Signs pass
+// through (the literal 1 - x.y of an off-sphere stage argument can be
+// negative).  The product must lie in the FP64 normal range, which four
+// pair denominators of magnitude >= 1e-70 or padding records satisfy.  A
+// zero denominator makes all four results non-finite.
 __device__ __forceinline__ void rcp4_64(double a, double b, double c, double d, double& ia, double& ib,
                                         double& ic, double& id) {
   const double ab = a * b, cd = c * d, P = ab * cd;
