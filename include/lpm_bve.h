@@ -334,7 +334,9 @@ int lpm_bve_last_pair_timing(lpm_bve_ctx* ctx, int* launches, double* mean_ms);
  * stop (0).  While on, steps run without graph replay. */
 int lpm_bve_enable_timing(lpm_bve_ctx* ctx, int on);
 
-/* Device memory a context needs for nv vertices and nactive active faces. */
+/* Device memory a context needs for nv vertices and nactive active faces
+ * at the default launch shape (state, source records, split partials).
+ * LPM_ERR_ARG for sizes lpm_bve_set_state would reject. */
 int lpm_bve_state_bytes(int64_t nv, int64_t nactive, int64_t* bytes);
 
 /* ------------------------------------------------------------------ */

@@ -3108,6 +3108,8 @@ int lpm_bve_autotune(lpm_bve_ctx* c, int prec, int scheme, int reps, int* best_R
 // Device memory one context needs for a problem of this size (bytes).
 int lpm_bve_state_bytes(int64_t nv, int64_t nactive, int64_t* bytes) {
   if (nv < 0 || nactive < 1 || !bytes) return fail(LPM_ERR_ARG, "lpm_bve_state_bytes: bad arguments");
+  if (nv + nactive > (int64_t)INT32_MAX / 8)
+    return fail(LPM_ERR_ARG, "lpm_bve_state_bytes: sizes out of range (the limit of lpm_bve_set_state)");
   const int64_t nt = nv + nactive;
   const Shape sh = choose_shape((int)nt, (int)nactive, 0, 0);
   const BufferPlan bp = plan_buffers((int)nt, sh);

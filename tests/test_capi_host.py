@@ -115,6 +115,13 @@ def test_state_bytes_count_the_default_split_partials(level, splits):
     assert E.state_bytes(nv, na) == state + records + splits * 4 * nt * 8 + 8 * 256
 
 
+def test_state_bytes_rejects_sizes_set_state_rejects():
+    from lpmkokkos_amd import ensemble as E
+    nv, na = 2 + 10 * 4 ** 14, 20 * 4 ** 14  # beyond lpm_bve_set_state's range
+    with pytest.raises(L.LpmError):
+        E.state_bytes(nv, na)
+
+
 def test_request_generator():
     from lpmkokkos_amd import serve as S
     reqs = S.make_requests(2000, rate=0.5, seed=4)
