@@ -215,8 +215,8 @@ def main() -> int:
     ap.add_argument("--ensemble-members", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; the multi-GPU sharding is a design "
-              "(target slices + per-stage source all-gather), not run here. Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU; sharding over several GPUs (target slices, per-stage "
+              "all-gather of the source records) is designed but not run. Use --gpus 1.", file=sys.stderr)
         return 2
 
     import lpmkokkos_amd as L

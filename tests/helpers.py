@@ -1,4 +1,4 @@
-"""Shared builders for the parity tests (oracle side is the checker)."""
+"""Shared builders for the parity tests (the oracle supplies the expected values)."""
 import numpy as np
 
 from oracle import oracle as O
