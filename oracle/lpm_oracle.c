@@ -2,9 +2,8 @@
  * lpm_oracle.c -- CPU oracle for the spherical barotropic-vorticity (BVE)
  * Biot-Savart hot path of pbosler/lpmKokkos.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
- * check and the CPU-baseline leg of bench.py load it to produce expected
- * values.  The product library (lpmkokkos_amd) never links, loads or calls
+ * TEST INFRASTRUCTURE ONLY.  tests/, the smoke check and the CPU-baseline
+ * leg of bench.py load it to produce expected values.  The product library (lpmkokkos_amd) never links, loads or calls
  * anything here, and nothing here runs on a GPU.
  *
  * Every function restates the reference's arithmetic in the reference's

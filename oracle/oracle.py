@@ -1,7 +1,7 @@
 """ctypes front end for the CPU oracle (liblpm_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
-baseline leg of bench.py as the checker.  The product package never imports
+baseline leg of bench.py for expected values.  The product package never imports
 this module.  Each wrapper names the reference function it restates.
 """
 from __future__ import annotations
