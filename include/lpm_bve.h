@@ -138,6 +138,24 @@ int lpm_poisson_y54(const double* xyz, int64_t n, double* f, double* u);
 int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, double* mean);
 
 /* ------------------------------------------------------------------ */
+/* Diagnostics (host).                                                 */
+/* ------------------------------------------------------------------ */
+
+/* ErrNorms(err, appx, exact, wt) for vector fields (src/LpmErrorNorms.cpp:15-45):
+ * err = appx - exact ([n][3]; err may be NULL), then, in index order,
+ * out = {l1, l2, linf}: l1 = sum|e|w / sum|x|w, l2 = sqrt(sum|e|^2 w / sum|x|^2 w),
+ * linf = max|e| / max|x|. */
+int lpm_err_norms_vec(double* err, const double* appx, const double* exact, const double* wt, int64_t n,
+                      double* out);
+/* ErrNorms(err, exact, wt) for a scalar error field err = appx - exact
+ * (src/LpmErrorNorms.cpp:56-70): l1 and l2 with |e| and |x|; linf = max e /
+ * max exact with SIGNED maxima, as the reference's reducer takes them. */
+int lpm_err_norms_scalar(const double* err, const double* exact, const double* wt, int64_t n, double* out);
+/* SphereVelocityTangentTestFunctor (src/LpmSphereTestKernels.hpp:11-25):
+ * udotx[i] = x_i . u_i for x, u [n][3]. */
+int lpm_velocity_tangent(const double* x, const double* u, int64_t n, double* udotx);
+
+/* ------------------------------------------------------------------ */
 /* Device functors (device pointers, asynchronous on `stream`, NULL =  */
 /* the null stream).  FP64, denominator 1 - x.y, as the reference.     */
 /* Scratch device memory is taken from a per-device workspace that     */

@@ -24,6 +24,7 @@
 #include <iostream>
 #include <cstdint>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -108,6 +109,52 @@ struct RossbyHaurwitzWave : public GalleryVorticity {
   std::string name() const override { return "rossby_haurwitz"; }
   Real phase_speed(Real omega) const { return (rh_n * (rh_n + 3) * rh_w - 2 * omega) / ((rh_n + 1) * (rh_n + 2)); }
 };
+
+// ErrNorms (src/LpmErrorNorms.hpp:81-130, src/LpmErrorNorms.cpp:7-70):
+// relative l1, l2 and linf errors with weights (areas), computed on the host
+// by lpm_err_norms_vec / lpm_err_norms_scalar in index order.
+struct ErrNorms {
+  Real l1 = 0, l2 = 0, linf = 0;
+  ErrNorms(const Real l_1, const Real l_2, const Real l_i) : l1(l_1), l2(l_2), linf(l_i) {}
+  // vector fields [n][3]: err (resized) receives appx - exact
+  ErrNorms(std::vector<Real>& err, const std::vector<Real>& appx, const std::vector<Real>& exact,
+           const std::vector<Real>& wt) {
+    const Index n = (Index)wt.size();
+    if ((Index)appx.size() < 3 * n || (Index)exact.size() < 3 * n)
+      throw std::invalid_argument("ErrNorms: vector fields need 3 values per weight");
+    err.resize(3 * n);
+    Real out[3];
+    check(lpm_err_norms_vec(err.data(), appx.data(), exact.data(), wt.data(), n, out), "ErrNorms");
+    l1 = out[0];
+    l2 = out[1];
+    linf = out[2];
+  }
+  // a scalar error field err = appx - exact
+  ErrNorms(const std::vector<Real>& err, const std::vector<Real>& exact, const std::vector<Real>& wt) {
+    const Index n = (Index)wt.size();
+    if ((Index)err.size() < n || (Index)exact.size() < n)
+      throw std::invalid_argument("ErrNorms: scalar fields need one value per weight");
+    Real out[3];
+    check(lpm_err_norms_scalar(err.data(), exact.data(), wt.data(), n, out), "ErrNorms");
+    l1 = out[0];
+    l2 = out[1];
+    linf = out[2];
+  }
+  std::string infoString(const std::string& label = "", const int tab_level = 0) const {
+    std::ostringstream ss;
+    for (int i = 0; i < tab_level; ++i) ss << "\t";
+    ss << label << " ErrNorms: l1 = " << l1 << " l2 = " << l2 << " linf = " << linf << "\n";
+    return ss.str();
+  }
+};
+
+// SphereVelocityTangentTestFunctor (src/LpmSphereTestKernels.hpp:11-25):
+// udotx[i] = x_i . u_i for positions and velocities [n][3].
+inline void velocity_tangent(std::vector<Real>& udotx, const std::vector<Real>& x, const std::vector<Real>& u) {
+  const Index n = (Index)std::min(x.size(), u.size()) / 3;
+  udotx.resize(n);
+  check(lpm_velocity_tangent(x.data(), u.data(), n, udotx.data()), "velocity_tangent");
+}
 
 template <typename SeedType>
 class BVESphere;

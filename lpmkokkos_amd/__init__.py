@@ -5,6 +5,6 @@ ABI of include/lpm_bve.h); this package is its Python front end.
 """
 from ._lib import (PREC_FP32C, PREC_FP64, SCHEME_LAGR, SCHEME_REF, VTK_LEGACY_ASCII, VTK_LEGACY_BINARY,  # noqa: F401
                    VTK_XML_ASCII, VTK_XML_BINARY, LpmError, lib)
-from .bve import (BVERK4, BVESphere, Context, IcosTriSphereSeed, MeshSeed, RossbyHaurwitzWave,  # noqa: F401
+from .bve import (BVERK4, BVESphere, Context, ErrNorms, IcosTriSphereSeed, MeshSeed, RossbyHaurwitzWave,  # noqa: F401
                   SolidBodyRotation, SpherePoisson, TreeMesh, VorticityInitialCondition, area_mean, gauss_params,
-                  poisson_y54, write_mesh_nc, write_mesh_vtk)
+                  poisson_y54, velocity_tangent, write_mesh_nc, write_mesh_vtk)

@@ -51,6 +51,9 @@ SIGNATURES = {
     "lpm_ic_eval": [_dp, C.c_int64, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dpn, _dpn,
                     C.c_double],
     "lpm_area_mean": [_dp, _dp, _bp, C.c_int64, C.POINTER(C.c_double)],
+    "lpm_err_norms_vec": [_dpn, _dp, _dp, _dp, C.c_int64, _dp],
+    "lpm_err_norms_scalar": [_dp, _dp, _dp, C.c_int64, _dp],
+    "lpm_velocity_tangent": [_dp, _dp, C.c_int64, _dp],
     "lpm_poisson_y54": [_dp, C.c_int64, _dpn, _dpn],
     "lpm_mesh_write_nc": [C.c_int, C.c_char_p, C.c_char_p],
     "lpm_mesh_write_vtk": [C.c_int, C.c_char_p, C.c_char_p, C.c_int],

@@ -195,6 +195,55 @@ def area_mean(fzeta, farea, fmask) -> float:
 # ---------------------------------------------------------------------------
 # device state
 # ---------------------------------------------------------------------------
+class ErrNorms:
+    """ErrNorms (src/LpmErrorNorms.hpp:81-130, src/LpmErrorNorms.cpp:7-70):
+    relative l1, l2 and linf errors with weights (areas), computed by the
+    library on the host in index order.  vector(): fields [n][3], the error
+    appx - exact is formed (and kept in .err); scalar(): a precomputed
+    error field, with the reference's signed maxima in linf."""
+
+    def __init__(self, l1: float, l2: float, linf: float, err=None):
+        self.l1, self.l2, self.linf = float(l1), float(l2), float(linf)
+        self.err = err
+
+    @classmethod
+    def vector(cls, appx, exact, wt) -> "ErrNorms":
+        a = np.ascontiguousarray(appx, np.float64).reshape(-1, 3)
+        x = np.ascontiguousarray(exact, np.float64).reshape(-1, 3)
+        w = np.ascontiguousarray(wt, np.float64)
+        if a.shape != x.shape or w.shape != (a.shape[0],):
+            raise ValueError("ErrNorms.vector: appx and exact [n][3], wt [n]")
+        err = np.empty_like(a)
+        out = np.zeros(3)
+        _lib.call("lpm_err_norms_vec", err.ctypes.data, a, x, w, a.shape[0], out)
+        return cls(*out, err=err)
+
+    @classmethod
+    def scalar(cls, err, exact, wt) -> "ErrNorms":
+        e = np.ascontiguousarray(err, np.float64)
+        x = np.ascontiguousarray(exact, np.float64)
+        w = np.ascontiguousarray(wt, np.float64)
+        if e.shape != x.shape or e.shape != w.shape or e.ndim != 1:
+            raise ValueError("ErrNorms.scalar: err, exact and wt [n]")
+        out = np.zeros(3)
+        _lib.call("lpm_err_norms_scalar", e, x, w, e.shape[0], out)
+        return cls(*out)
+
+    def infoString(self, label: str = "", tab_level: int = 0) -> str:
+        return "\t" * tab_level + f"{label} ErrNorms: l1 = {self.l1:g} l2 = {self.l2:g} linf = {self.linf:g}\n"
+
+
+def velocity_tangent(x, u) -> np.ndarray:
+    """SphereVelocityTangentTestFunctor (src/LpmSphereTestKernels.hpp:11-25): x_i . u_i."""
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    u = np.ascontiguousarray(u, np.float64).reshape(-1, 3)
+    if x.shape != u.shape:
+        raise ValueError("velocity_tangent: x and u [n][3]")
+    out = np.zeros(x.shape[0])
+    _lib.call("lpm_velocity_tangent", x, u, x.shape[0], out)
+    return out
+
+
 class Context:
     """One lpm_bve_ctx: a particle state resident in device memory."""
 

@@ -12,9 +12,12 @@
 //    registers for the whole source sweep.
 //  * Sources are compacted active faces in the tree order (which is spatial:
 //    consecutive 4^k leaves are one level-(L-k) triangle), padded to whole
-//    tiles of kTile.  A wave reads a source with a uniform address, so the
-//    record is fetched once per wave into scalar registers through the scalar
-//    cache and broadcast to all lanes -- no LDS staging, no barriers.
+//    tiles of kTile.  By default a workgroup stages its source tiles through
+//    LDS, double-buffered: every lane copies one record of tile t+1 (one
+//    coalesced load) while the waves sum tile t, with a barrier per tile;
+//    waves read the records with uniform-address (broadcast) LDS loads.
+//    The alternative staging reads each record with a uniform address
+//    straight from memory, which the compiler issues as scalar loads.
 //  * The source range is cut into `splits` contiguous tile ranges
 //    (grid.y); each split writes FP64 partial sums that a finalize kernel
 //    adds in split order, so results do not depend on scheduling.

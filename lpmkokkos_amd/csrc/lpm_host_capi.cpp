@@ -1,4 +1,6 @@
 // C ABI for the host-side pieces: mesh build, initial conditions, errors.
+#include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -231,6 +233,66 @@ int lpm_area_mean(const double* fzeta, const double* farea, const uint8_t* fmask
   }
   if (den == 0.0) return fail(LPM_ERR_ARG, "lpm_area_mean: zero total area");
   *mean = num / den;
+  return ok();
+}
+
+// ErrNorms::compute for vector fields (src/LpmErrorNorms.cpp:15-45): the
+// error appx - exact (written to err when non-null), then in index order
+// l1 = sum |e| w / sum |x| w, l2 = sqrt(sum |e|^2 w / sum |x|^2 w) and
+// linf = max |e| / max |x|, with |v| = sqrt(v.v) (SphereGeometry::mag).
+int lpm_err_norms_vec(double* err, const double* appx, const double* exact, const double* wt, int64_t n,
+                      double* out) {
+  if (n < 1 || !appx || !exact || !wt || !out) return fail(LPM_ERR_ARG, "lpm_err_norms_vec: bad arguments");
+  double l1n = 0, l1d = 0, l2n = 0, l2d = 0, lin = -DBL_MAX, lid = -DBL_MAX;
+  for (int64_t i = 0; i < n; ++i) {
+    const double e[3] = {appx[3 * i] - exact[3 * i], appx[3 * i + 1] - exact[3 * i + 1],
+                         appx[3 * i + 2] - exact[3 * i + 2]};
+    if (err)
+      for (int k = 0; k < 3; ++k) err[3 * i + k] = e[k];
+    const double* x = exact + 3 * i;
+    const double em = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    const double xm = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    l1n += em * wt[i];
+    l1d += xm * wt[i];
+    l2n += (em * em) * wt[i];
+    l2d += (xm * xm) * wt[i];
+    if (em > lin) lin = em;
+    if (xm > lid) lid = xm;
+  }
+  out[0] = l1n / l1d;
+  out[1] = std::sqrt(l2n / l2d);
+  out[2] = lin / lid;
+  return ok();
+}
+
+// ErrNorms::compute for a scalar error field err = appx - exact
+// (src/LpmErrorNorms.cpp:56-70): l1 and l2 as above with |e| and |x|; the
+// maxima are signed, as the reference's reducer takes them (max e over max
+// exact, from the max identity -DBL_MAX).
+int lpm_err_norms_scalar(const double* err, const double* exact, const double* wt, int64_t n, double* out) {
+  if (n < 1 || !err || !exact || !wt || !out) return fail(LPM_ERR_ARG, "lpm_err_norms_scalar: bad arguments");
+  double l1n = 0, l1d = 0, l2n = 0, l2d = 0, lin = -DBL_MAX, lid = -DBL_MAX;
+  for (int64_t i = 0; i < n; ++i) {
+    const double e = err[i], x = exact[i];
+    l1n += std::fabs(e) * wt[i];
+    l1d += std::fabs(x) * wt[i];
+    l2n += (e * e) * wt[i];
+    l2d += (x * x) * wt[i];
+    lin = e > lin ? e : lin;
+    lid = x > lid ? x : lid;
+  }
+  out[0] = l1n / l1d;
+  out[1] = std::sqrt(l2n / l2d);
+  out[2] = lin / lid;
+  return ok();
+}
+
+// SphereVelocityTangentTestFunctor (src/LpmSphereTestKernels.hpp:11-25):
+// udotx[i] = x_i . u_i (SphereGeometry::dot, left to right).
+int lpm_velocity_tangent(const double* x, const double* u, int64_t n, double* udotx) {
+  if (n < 0 || (n && (!x || !u || !udotx))) return fail(LPM_ERR_ARG, "lpm_velocity_tangent: bad arguments");
+  for (int64_t i = 0; i < n; ++i)
+    udotx[i] = x[3 * i] * u[3 * i] + x[3 * i + 1] * u[3 * i + 1] + x[3 * i + 2] * u[3 * i + 2];
   return ok();
 }
 

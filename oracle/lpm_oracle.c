@@ -29,6 +29,7 @@
  *   ICs / PRNG    -- published SplitMix64 / xoshiro256** output vectors,
  *                    closed-form RH values.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -799,11 +800,14 @@ ORC_EXPORT void orc_ic_eval(const double* x, int n, double* zeta, double sbr, in
 }
 
 /* ------------------------------------------------------------------ */
-/* error norms: src/LpmErrorNorms.cpp:26-70                            */
+/* error norms: src/LpmErrorNorms.cpp:26-70 (ErrNorms::compute, the       */
+/* ENormScalar reducer of src/LpmErrorNorms.hpp:13-30)                 */
 /* ------------------------------------------------------------------ */
-/* Vector norms: out = {l1, l2, linf}.  linf = max|e| / max|exact|. */
+/* Vector norms (:26-45): out = {l1, l2, linf} with e = appx - exact (:16-24),
+ * l1 = sum |e| w / sum |x| w, l2 = sqrt(sum |e|^2 w / sum |x|^2 w),
+ * linf = max |e| / max |x|. */
 ORC_EXPORT void orc_err_norms_vec(const double* appx, const double* exact, const double* wt, int n, double* out) {
-  double l1n = 0, l1d = 0, l2n = 0, l2d = 0, lin = 0, lid = 0;
+  double l1n = 0, l1d = 0, l2n = 0, l2d = 0, lin = -DBL_MAX, lid = -DBL_MAX;
   for (int i = 0; i < n; ++i) {
     double e[3] = {appx[3 * i] - exact[3 * i], appx[3 * i + 1] - exact[3 * i + 1], appx[3 * i + 2] - exact[3 * i + 2]};
     const double em = g_mag(e), xm = g_mag(&exact[3 * i]);
@@ -818,18 +822,20 @@ ORC_EXPORT void orc_err_norms_vec(const double* appx, const double* exact, const
   out[1] = sqrt(l2n / l2d);
   out[2] = lin / lid;
 }
-/* Scalar norms with |e| in the maximum (the reference's scalar linf takes a
- * signed maximum, src/LpmErrorNorms.cpp:64-65; magnitudes are used here). */
+/* Scalar norms (:47-70): l1 and l2 as above with |e| and |x|; the maxima
+ * are SIGNED (:64-65: linfnum = max e, linfdenom = max exact, starting
+ * from the max-reduction identity -DBL_MAX), so an error that is negative
+ * everywhere gives linf <= 0. */
 ORC_EXPORT void orc_err_norms_scalar(const double* appx, const double* exact, const double* wt, int n, double* out) {
-  double l1n = 0, l1d = 0, l2n = 0, l2d = 0, lin = 0, lid = 0;
+  double l1n = 0, l1d = 0, l2n = 0, l2d = 0, lin = -DBL_MAX, lid = -DBL_MAX;
   for (int i = 0; i < n; ++i) {
-    const double e = fabs(appx[i] - exact[i]), x = fabs(exact[i]);
-    l1n += e * wt[i];
-    l1d += x * wt[i];
-    l2n += e * e * wt[i];
-    l2d += x * x * wt[i];
-    if (e > lin) lin = e;
-    if (x > lid) lid = x;
+    const double e = appx[i] - exact[i], x = exact[i];
+    l1n += fabs(e) * wt[i];
+    l1d += fabs(x) * wt[i];
+    l2n += (e * e) * wt[i];
+    l2d += (x * x) * wt[i];
+    lin = e > lin ? e : lin;
+    lid = x > lid ? x : lid;
   }
   out[0] = l1n / l1d;
   out[1] = sqrt(l2n / l2d);

@@ -146,3 +146,43 @@ def test_poisson_y54_matches_oracle(oracle_mod):
         of, ou = oracle_mod.sphharm54(x)
         assert np.all(np.abs(f - of) <= 2 * np.spacing(np.abs(of)))
         assert np.array_equal(u, ou)
+
+
+def test_err_norms_match_oracle(oracle_mod, omesh):
+    """The product's ErrNorms (host) equals the oracle's restatement of
+    src/LpmErrorNorms.cpp:15-70 bit for bit, on the solid-body velocity of a
+    level-3 mesh with area weights, and the closed forms hold."""
+    O = oracle_mod
+    m = omesh(3)
+    lv = m.fmask == 0
+    x = m.fx[lv]
+    Om = 2 * np.pi
+    exact = np.stack([-Om * x[:, 1], Om * x[:, 0], 0 * x[:, 0]], axis=1)
+    rng = np.random.default_rng(5)
+    appx = exact + 1e-3 * rng.normal(size=exact.shape)
+    w = m.farea[lv]
+    n = L.ErrNorms.vector(appx, exact, w)
+    assert [n.l1, n.l2, n.linf] == list(O.err_norms_vec(appx, exact, w))
+    assert np.array_equal(n.err, appx - exact)
+    s_ex = exact[:, 0]
+    s_app = appx[:, 0]
+    ns = L.ErrNorms.scalar(s_app - s_ex, s_ex, w)
+    assert [ns.l1, ns.l2, ns.linf] == list(O.err_norms_scalar(s_app, s_ex, w))
+    c = L.ErrNorms.vector(1.25 * exact, exact, w)
+    assert abs(c.l1 - 0.25) < 1e-15 and abs(c.l2 - 0.25) < 1e-15 and abs(c.linf - 0.25) < 1e-15
+    xs = rng.uniform(1.0, 2.0, 50)
+    neg = L.ErrNorms.scalar(-0.5 * xs, xs, np.ones(50))
+    assert abs(neg.linf + 0.5 * xs.min() / xs.max()) < 1e-15  # signed maxima, as the reference
+    assert n.infoString("velocity").startswith("velocity ErrNorms: l1 = ")
+    with pytest.raises(ValueError):
+        L.ErrNorms.scalar(np.zeros(3), np.zeros(4), np.ones(3))
+
+
+def test_velocity_tangent():
+    """SphereVelocityTangentTestFunctor: u.x, zero for Omega z x x."""
+    m = L.TreeMesh(2)
+    u = np.stack([-m.vx[:, 1], m.vx[:, 0], 0 * m.vx[:, 0]], axis=1)
+    d = L.velocity_tangent(m.vx, u)
+    assert np.abs(d).max() < 1e-15
+    d2 = L.velocity_tangent(m.vx, m.vx)
+    assert np.abs(d2 - 1).max() < 1e-15

@@ -309,3 +309,72 @@ def test_reference_scheme_sbr_instability(oracle_mod, omesh):
         ex = np.stack([lm.vx[:, 0] * c - lm.vx[:, 1] * s, lm.vx[:, 1] * c + lm.vx[:, 0] * s, lm.vx[:, 2]], 1)
         errs.append(np.abs(st.vx - ex).max())
     assert errs[0] > 1.0 and errs[1] < 1e-2, errs
+
+
+@pytest.mark.parametrize("level", [2, 3, 4, 5])
+def test_chord_forms_equal_dot_forms(oracle_mod, omesh, level):
+    """The chord-denominator sums (|y - x|^2 / 2, the Lagrangian scheme's form)
+    equal the reference's literal 1 - x.y sums (src/LpmBVEKernels.hpp:21-44)
+    for particles on the unit sphere.  The field carries a constant offset,
+    so total circulation is nonzero and a constant error in psi (a dropped
+    or doubled ln 2 in the chord form) cannot cancel; vertex and active-face
+    targets (face targets skip their own source)."""
+    O = oracle_mod
+    m = omesh(level)
+    z = O.ic_eval(m.fx, rh_n=4, rh_w=0.1, rh_K=0.1) + 0.7
+    lv = m.fmask == 0
+    circ = float((z * m.farea)[lv].sum())
+    assert abs(circ) > 1.0
+    for tx, col, rows in ((m.vx, False, slice(None)), (m.fx, True, lv)):
+        u = O.velocity(tx, m.fx, z, m.farea, m.fmask, col)[rows]
+        uc = O.velocity_chord(tx, m.fx, z, m.farea, m.fmask, col)[rows]
+        p = O.streamfn(tx, m.fx, z, m.farea, m.fmask, col)[rows]
+        pc = O.streamfn_chord(tx, m.fx, z, m.farea, m.fmask, col)[rows]
+        assert np.abs(uc - u).max() <= 1e-12 * np.abs(u).max()
+        assert np.abs(pc - p).max() <= 1e-12 * np.abs(p).max()
+    # a constant psi offset of ln2 * sum(q) (q = -zeta A / 4 pi) would be this large
+    assert math.log(2) * abs(circ) / (4 * PI) > 1e-3 * np.abs(p).max()
+
+
+def test_chord_stride_rows(oracle_mod, omesh):
+    """velocity_chord with a stride fills every stride-th row and leaves the
+    others zero; the filled rows equal the full evaluation's."""
+    O = oracle_mod
+    m = omesh(3)
+    z = O.ic_eval(m.fx, rh_n=4, rh_w=0.1, rh_K=0.1)
+    full = O.velocity_chord(m.vx, m.fx, z, m.farea, m.fmask, False)
+    part = O.velocity_chord(m.vx, m.fx, z, m.farea, m.fmask, False, stride=7)
+    assert np.array_equal(part[::7], full[::7])
+    mask = np.ones(len(part), bool)
+    mask[::7] = False
+    assert np.all(part[mask] == 0.0)
+
+
+def test_err_norms_closed_forms(oracle_mod):
+    """ErrNorms (src/LpmErrorNorms.cpp:26-70) on fields whose norms are known
+    in closed form: appx = (1 + c) exact gives l1 = l2 = linf = |c| for vector
+    fields; scalar fields take signed maxima (:64-65), so a negative error
+    everywhere gives linf = c min(exact) / max(exact) < 0 (not |c|); a single
+    wrong entry with weights gives the weighted ratios."""
+    O = oracle_mod
+    rng = np.random.default_rng(3)
+    n = 500
+    ex = rng.normal(size=(n, 3))
+    w = rng.uniform(0.5, 2.0, n)
+    for c in (0.25, -0.125):
+        l1, l2, li = O.err_norms_vec((1 + c) * ex, ex, w)
+        assert abs(l1 - abs(c)) < 1e-15 and abs(l2 - abs(c)) < 1e-15 and abs(li - abs(c)) < 1e-15
+    xs = rng.uniform(1.0, 3.0, n)
+    l1, l2, li = O.err_norms_scalar(1.5 * xs, xs, w)
+    assert abs(l1 - 0.5) < 1e-15 and abs(l2 - 0.5) < 1e-15 and abs(li - 0.5) < 1e-15
+    l1, l2, li = O.err_norms_scalar(0.75 * xs, xs, w)
+    assert abs(l1 - 0.25) < 1e-15 and abs(l2 - 0.25) < 1e-15
+    assert abs(li - (-0.25 * xs.min() / xs.max())) < 1e-15
+    # one wrong entry: e_k = d at index k, exact = 1 everywhere
+    one = np.ones(n)
+    app = one.copy()
+    app[17] += 0.5
+    l1, l2, li = O.err_norms_scalar(app, one, w)
+    assert abs(l1 - 0.5 * w[17] / w.sum()) < 1e-15
+    assert abs(l2 - math.sqrt(0.25 * w[17] / w.sum())) < 1e-15
+    assert abs(li - 0.5) < 1e-15
