@@ -11,12 +11,22 @@ solve, which is the next step's stage 1).
 
 value  = FP32C pair interactions per second over the K timed steps, whole
          job, state resident in HBM, steps replayed from a captured hipGraph.
-Also reported: the FP64 path's rate over 10 steps from the same initial
-state, the FP32C/FP64 speed-up, FP32C error against FP64 (positions after
-those 10 steps; velocity from the step kernel and stream function at the
-final FP64 state), the FP64 reference scheme's rate, the pair kernel's
-roofline position (HIP events around the kernel on its own stream), and the
-CPU baseline: the oracle's reference-structured loop on a bounded sample.
+Also reported, in the same line:
+  * the FP64 path's rate over 10 steps from the same initial state, the
+    FP32C/FP64 speed-up beside its instruction-issue bound, FP32C error
+    against FP64 (positions after those 10 steps; velocity from the step
+    kernel and stream function at the final FP64 state), the FP64 reference
+    scheme's rate, and the pair kernel's roofline position (HIP events around
+    the kernel on its own stream);
+  * one timed FP64 reference RK4 step at level 7 (the same field);
+  * the batching configuration in full (64 members, levels 3-7, 10 FP32C
+    steps): ensemble rate, the members' sequential rate, and whether every
+    member equals its run alone bit for bit;
+  * a 40-request sample of the serving configuration (levels 4-8 with the
+    configuration's probabilities, 20 steps each): p50/p99 overall and by
+    level, aggregate pair interactions/s;
+  * the CPU baseline: the oracle's reference-structured loop on the host's
+    physical cores (C0 and C1 in full, C2 on a 1/64 target subset).
 
 Pairs per evaluation = (nv + nl) * nl - nl (target x active source, self
 excluded).  Algorithmic flops per pair = 15 (d = y - x: 3, |d|^2: 5,
@@ -46,6 +56,9 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # from two rocprofv3 --pmc runs, each scaled by the same counter's reading for a device copy of
 # known size
 TRAFFIC_BYTES_PER_LAUNCH = 1.4037e9 + 1.1347e9
+# FP32C/FP64 speed-up per step allowed by the pair loops' instruction issue
+# (22.2 vs 49.6 SIMD cycles per 64 pairs, measured issue costs)
+FP32C_VS_FP64_ISSUE_BOUND = 49.6 / 22.2
 DT = 0.005
 OMEGA = 2 * np.pi
 
@@ -67,37 +80,19 @@ def build_problem(level: int, seed: int = 1):
     return mesh, fx, farea, vz - mu, fz - mu
 
 
-def cpu_baseline(mesh, fz_tree, budget_s: float):
-    """Reference-structured CPU loop (oracle): per target, a sequential FP64
-    sweep over all tree faces with the mask, literal 1 - x.y.  Timed on a
-    strided sample of face targets sized to ~budget_s seconds."""
-    from oracle import oracle as O
-    O.lib()
-    nt = mesh.nfaces
-    nl = int((mesh.fmask == 0).sum())
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-
-    def run(stride):
-        t0 = time.perf_counter()
-        O.velocity(mesh.fx, mesh.fx, fz_tree, mesh.farea, mesh.fmask, True, stride=stride)
-        return time.perf_counter() - t0, len(range(0, nt, stride))
-
-    dt, n = run(max(1, nt // (64 * threads)))  # calibration: 64 targets per thread
-    per_target = dt / n
-    want = max(threads, int(budget_s / per_target))
-    stride = max(1, nt // want)
-    dt, n = run(stride)
-    iters = n * mesh.nfaces
-    pairs = n * nl
-    return {
-        "value": pairs / dt,
-        "unit": "pair-interactions/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{n} face targets (every {stride}th of {nt} tree faces) x {mesh.nfaces} tree-face sources "
-                  f"({nl} active), FP64, {dt:.1f} s; reference loop iterations/s = {iters / dt:.4g}",
-    }
+def cpu_baseline(timeout_s: float = 240.0):
+    """The oracle's reference-structured loop (oracle/cpu_baseline.py) on the
+    host's physical cores, in a child process so that its OpenMP thread
+    binding leaves this process's threads alone: C0 and C1 in full, C2 on a
+    1/64 strided target subset."""
+    import subprocess
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py")], env=env,
+                       capture_output=True, text=True, timeout=timeout_s)
+    sys.stderr.write(r.stderr)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed ({r.returncode})")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def serving_form(nreq: int, max_level: int, load: float = 0.8, steps: int = 20, seed: int = 4):
@@ -113,85 +108,123 @@ def serving_form(nreq: int, max_level: int, load: float = 0.8, steps: int = 20, 
     mean_service = sum(probs[l] * svc[l] for l in probs)
     rate = load / mean_service
     reqs = S.make_requests(nreq, rate, seed=seed, level_probs=probs, steps=steps)
-    srv = S.Server(budget_bytes=200e9)
+    srv = S.Server(budget_bytes=200e9, log=log)
     res = srv.run(reqs)
     srv.close()
+    levels = res["levels"]
     res.update({"arrival_rate_per_s": rate, "load": load, "level_probs": probs,
                 "service_s_by_level": {int(k): v for k, v in svc.items()}, "steps_per_request": steps,
-                "sample": f"{nreq} requests (levels <= {max_level}) of the 2000-request serving configuration"})
+                "sample": f"{nreq} requests of the 2000-request serving configuration (seed {seed}; levels "
+                          + ", ".join(f"{l}: {n}" for l, n in sorted(levels.items()))
+                          + f"), Poisson arrivals at {load:.0%} of the measured capacity; p99 of {nreq} "
+                          "completions is close to their maximum"})
     return res
 
 
-def ensemble_form(args) -> int:
-    """Batching configuration: 64 members, levels uniform on 3..7, RH wave
-    (K x U(0.95, 1.05)) + 8 Gaussian vortices, 10 FP32C steps, one persistent
-    pair kernel per evaluation, step graph replayed; members checked against
-    the same problems run alone."""
+def ensemble_run(nmembers: int = 64, nsteps: int = 10) -> dict:
+    """Batching configuration: 64 members, levels uniform on 3..7 (seed 3), RH
+    wave (K x U(0.95, 1.05)) + 8 Gaussian vortices, 10 FP32C steps, one
+    persistent pair kernel per evaluation, step graph replayed, launch shapes
+    from the autotuner.  Every member is then advanced alone by the same 10
+    steps (one at a time, as a caller without batching would run them: the
+    sequential rate) and compared with its ensemble result bit for bit."""
     import lpmkokkos_amd as L
     from lpmkokkos_amd import ensemble as E
-    specs = E.ensemble_config(seed=3, n=args.ensemble_members)
+    specs = E.ensemble_config(seed=3, n=nmembers)
     t0 = time.perf_counter()
     members = [E.make_member(s) for s in specs]
-    log(f"[bench] ensemble: {len(members)} members built in {time.perf_counter() - t0:.1f} s")
     tuned = {}
     for s, c in zip(specs, members):
         if s.level not in tuned:
             tuned[s.level] = E.autotune(c, reps=1)
-    log(f"[bench] autotuned shapes (R, splits, ms): {tuned}")
+    log(f"[bench] ensemble: {len(members)} members built and shapes autotuned in {time.perf_counter() - t0:.1f} s: "
+        f"{tuned}")
     ens = E.Ensemble(0)
     for c in members:
         ens.add(c)
     ens.advance(DT, OMEGA, nsteps=1)  # warm: entry velocities, plan, capture
     ens.synchronize()
-    nsteps = 10  # the batching configuration's step count
     t1 = time.perf_counter()
     ens.advance(DT, OMEGA, nsteps=nsteps)
     ens.synchronize()
     dt_ens = time.perf_counter() - t1
     pairs_step = sum(4.0 * ((2 + 10 * 4 ** s.level + 20 * 4 ** s.level) * 20 * 4 ** s.level - 20 * 4 ** s.level)
                      for s in specs)
-    # the same steps run member by member (each alone, own graph)
-    alone = [E.make_member(s) for s in specs[:8]]
-    for c in alone:
-        c.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1 + nsteps)
-    same = all(np.array_equal(a.get_state()[k], b.get_state()[k])
-               for a, b in zip(alone, members[:8]) for k in ("vx", "fx", "vu", "fu", "fzeta"))
-    t2 = time.perf_counter()
-    seq = [E.make_member(s) for s in specs]
-    for c in seq:
-        c.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1)
-    for c in seq:
-        c.synchronize()
-    t3 = time.perf_counter()
-    for c in seq:  # one problem at a time, as a caller without batching would run them
-        c.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=nsteps)
-        c.synchronize()
-    dt_seq = time.perf_counter() - t3
+    log(f"[bench] ensemble: {1e3 * dt_ens / nsteps:.1f} ms/step, {pairs_step * nsteps / dt_ens:.4e} pairs/s")
+    dt_seq = 0.0
+    same = 0
+    for s, b in zip(specs, members):
+        a = E.make_member(s)
+        a.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=1)  # the warm step, as the ensemble's
+        a.synchronize()
+        t = time.perf_counter()
+        a.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=nsteps)
+        a.synchronize()
+        dt_seq += time.perf_counter() - t
+        sa, sb = a.get_state(), b.get_state()
+        same += all(np.array_equal(sa[k], sb[k]) for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"))
+        a.close()
+    log(f"[bench] ensemble: members alone {pairs_step * nsteps / dt_seq:.4e} pairs/s; {same}/{len(specs)} "
+        "bitwise equal")
+    ens.close()
+    for c in members:
+        c.close()
+    return {
+        "workload": f"{len(specs)} members, levels uniform on 3..7 (seed 3), RH n=4 K x U(0.95,1.05) "
+                    "+ 8 Gaussian vortices N(0,0.05) sigma 0.15, Lagrangian RK4 dt=0.005, FP32C",
+        "levels": {int(l): sum(1 for s in specs if s.level == l) for l in sorted({s.level for s in specs})},
+        "steps": nsteps,
+        "pairs_per_step": pairs_step,
+        "ms_per_step": 1e3 * dt_ens / nsteps,
+        "pairs_per_s": pairs_step * nsteps / dt_ens,
+        "members_sequential_pairs_per_s": pairs_step * nsteps / dt_seq,
+        "speedup_vs_sequential": dt_seq / dt_ens,
+        "members_bitwise_equal_alone": f"{same}/{len(specs)}",
+        "all_members_bitwise_equal_alone": same == len(specs),
+        "autotuned_shapes": {int(k): list(v) for k, v in tuned.items()},
+    }
+
+
+def ensemble_form(args) -> int:
+    r = ensemble_run(args.ensemble_members)
     out = {
         "metric": "ensemble pair-interactions/s (batching form)",
-        "value": pairs_step * nsteps / dt_ens,
+        "value": r["pairs_per_s"],
         "unit": "pair-interactions/s",
         "n_gpus": 1,
-        "steps": nsteps,
-        "ms_per_step": 1e3 * dt_ens / nsteps,
+        "steps": r["steps"],
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{len(specs)} members, levels uniform on 3..7 (seed 3), RH n=4 K x U(0.95,1.05) "
-                               "+ 8 Gaussian vortices N(0,0.05) sigma 0.15, Lagrangian RK4 dt=0.005",
-                   "levels": {int(l): sum(1 for s in specs if s.level == l) for l in sorted({s.level for s in specs})},
-                   "pairs_per_step": pairs_step},
-        "members_sequential_pairs_per_s": pairs_step * nsteps / dt_seq,
-        "ensemble_speedup_vs_sequential": dt_seq / dt_ens,
-        "first8_members_bitwise_equal_alone": bool(same),
-        "autotuned_shapes": {int(k): v for k, v in tuned.items()},
-        "build_s": t2 - t0,
+        "config": {"workload": r["workload"], "levels": r["levels"], "pairs_per_step": r["pairs_per_step"]},
+        "ensemble": r,
     }
     print(json.dumps(out), flush=True)
-    ens.close()
-    for c in members + alone + seq:
-        c.close()
     return 0
+
+
+def c1_gpu_step(level: int = 7) -> dict:
+    """Plain FP64 BVE at level 7 (163,842 vertices, 327,680 faces), the
+    32-Gaussian-vortex field of seed 1: one timed RK4 step of the reference
+    scheme (literal 1 - x.y, dt = 0.01) after one untimed step."""
+    import lpmkokkos_amd as L
+    mesh, fx, farea, vz, fz = build_problem(level)
+    nv, nl = mesh.nverts, len(fx)
+    c = L.Context(0)
+    c.set_state(mesh.vx, vz, fx, fz, farea, np.zeros(nl, np.uint8))
+    c.solve(L.PREC_FP64, L.SCHEME_REF)
+    c.advance(0.01, OMEGA, L.SCHEME_REF, L.PREC_FP64, nsteps=1)  # capture
+    c.synchronize()
+    t = time.perf_counter()
+    c.advance(0.01, OMEGA, L.SCHEME_REF, L.PREC_FP64, nsteps=1)
+    c.synchronize()
+    dt = time.perf_counter() - t
+    c.close()
+    pairs = 4 * ((nv + nl) * nl - nl)
+    return {"workload": f"icosahedral level {level}: {nv} vertices, {nl} faces; 32 Gaussian vortices (seed 1); "
+                        "reference RK4 step, FP64, dt=0.01",
+            "ms_per_step": 1e3 * dt, "pairs_per_step": pairs, "pairs_per_s": pairs / dt}
 
 
 def main() -> int:
@@ -202,16 +235,17 @@ def main() -> int:
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--compare-steps", type=int, default=10,
                     help="FP64 and FP32C steps from the same state: FP64 rate and FP32C error (0: skip)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--targets-per-lane", type=int, default=0)
     ap.add_argument("--splits", type=int, default=0)
     ap.add_argument("--near-factor", type=float, default=None)
     ap.add_argument("--staging", choices=["lds", "scalar"], default=None)
     ap.add_argument("--form", choices=["precision", "ensemble", "serving"], default="precision")
-    ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-max-level", type=int, default=7)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-max-level", type=int, default=8)
     ap.add_argument("--ensemble-members", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -330,12 +364,19 @@ def main() -> int:
         c64.close()
     ctx.close()
 
+    c1 = None
+    if not args.no_c1:
+        c1 = c1_gpu_step()
+        log(f"[bench] level-7 FP64 reference RK4 step: {c1['ms_per_step']:.1f} ms, {c1['pairs_per_s']:.4e} pairs/s")
+
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = ensemble_run(args.ensemble_members)
+
     cpu = None
     if not args.no_cpu:
-        mesh_z = np.zeros(mesh.nfaces)
-        mesh_z[mesh.fmask == 0] = fz
-        cpu = cpu_baseline(mesh, mesh_z, args.cpu_seconds)
-        log(f"[bench] CPU baseline: {cpu['value']:.4e} pairs/s on {cpu['cores']} threads")
+        cpu = cpu_baseline()
+        log(f"[bench] CPU baseline: {cpu['value']:.4e} pairs/s on {cpu['cores']} threads ({cpu['cpu_model']})")
 
     serving = None
     if args.serve_requests > 0:
@@ -366,6 +407,11 @@ def main() -> int:
         "fp64_pairs_per_s": rate64,
         "fp64_reference_scheme_pairs_per_s": rate64_ref,
         "fp32c_vs_fp64_speedup": rate / rate64 if rate64 else None,
+        "fp32c_vs_fp64_speedup_bound": FP32C_VS_FP64_ISSUE_BOUND,
+        "fp32c_vs_fp64_speedup_bound_note": "instruction-issue limit of the two pair loops: FP64 FMA issues at half "
+                                            "the FP32 rate, 49.6 vs 22.2 SIMD cycles per 64 pairs",
+        "c1_fp64_reference_step": c1,
+        "ensemble": ensemble,
         "fp32c_error_vs_fp64": err,
         "roofline": {
             "bound": "valu",

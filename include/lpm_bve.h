@@ -157,9 +157,11 @@ int lpm_velocity_tangent(const double* x, const double* u, int64_t n, double* ud
 
 /* ------------------------------------------------------------------ */
 /* Device functors (device pointers, asynchronous on `stream`, NULL =  */
-/* the null stream).  FP64, denominator 1 - x.y, as the reference.     */
-/* Scratch device memory is taken from a per-device workspace that     */
-/* grows on demand; calls on one device must not overlap.              */
+/* the null stream: a call queues its kernels and returns).  FP64,     */
+/* denominator 1 - x.y, as the reference.  Scratch device memory comes */
+/* from a per-device workspace that grows on demand (growing waits for */
+/* the previous call and allocates); each call waits on the device for */
+/* the previous call's kernels before reusing it, on any stream.       */
 /* ------------------------------------------------------------------ */
 
 /* BVEVertexVelocity (src/LpmBVEKernels.hpp:163-186): u[nv][3] at targets

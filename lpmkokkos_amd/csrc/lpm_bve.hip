@@ -82,13 +82,16 @@ struct Src64 {
 };
 
 // Dot-form records.  Source j is target smap[j]; its area is area[smap[j]].
-// Padding records sit at the origin with q = 0 (denominator 1).
+// Padding records sit at the origin with q = 0 (denominator 1).  ns_dev
+// (may be null) holds the source count on the device, at most ns.
 __global__ __launch_bounds__(kBlock) void k_prep_f64_dot(Src64* __restrict__ src, Pos3 pos,
                                                          const double* __restrict__ zeta,
                                                          const double* __restrict__ area,
-                                                         const int* __restrict__ smap, int ns, int ns_pad) {
+                                                         const int* __restrict__ smap, int ns, int ns_pad,
+                                                         const int* __restrict__ ns_dev) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= ns_pad) return;
+  if (ns_dev) ns = min(ns, *ns_dev);
   Src64 r;
   if (j < ns) {
     const int t = smap[j];
@@ -305,6 +308,9 @@ struct PairArgs {
   double* part;          // [splits][kNC][nt_pad]
   int nt_pad;
   float near_factor;  // FP32C: near-form threshold in tile radii
+  // FP64 kernels: the source count on the device (may be null); tiles past
+  // it hold only padding records and are skipped
+  const int* ns_dev;
 };
 
 template <int R>
@@ -442,8 +448,9 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
 #pragma unroll
   for (int r = 0; r < R; ++r)
     for (int c = 0; c < kNC; ++c) S[r][c] = 0.0;
+  const int ntiles = a.ns_dev ? min(a.ntiles, (*a.ns_dev + kTile - 1) / kTile) : a.ntiles;
   const int t0 = by * a.tiles_per_split;
-  const int t1 = min(a.ntiles, t0 + a.tiles_per_split);
+  const int t1 = min(ntiles, t0 + a.tiles_per_split);
   if constexpr (LDS) {
     if (t0 < t1) sh[0][threadIdx.x] = src[(long)t0 * kTile + threadIdx.x];
     __syncthreads();
@@ -1338,6 +1345,8 @@ struct EvalRequest {
   int form;  // Form (FP64 only)
   bool want_psi;
   Shape shape;
+  // FP64 dot form: the source count on the device (at most ns), or null
+  const int* ns_dev = nullptr;
 };
 
 template <int R, bool LDS>
@@ -1385,6 +1394,7 @@ PairArgs make_pair_args(const EvalRequest& q, const EvalBuffers& b) {
   pa.part = b.part;
   pa.nt_pad = q.nt;
   pa.near_factor = q.shape.near_factor;
+  pa.ns_dev = q.ns_dev;
   return pa;
 }
 
@@ -1398,7 +1408,7 @@ int enqueue_prep(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
     if (q.want_psi) hipLaunchKernelGGL(k_qsum, dim3(1), dim3(kBlock), 0, st, b.ctr, sh.ntiles, b.qtot);
   } else if (q.form == FORM_DOT) {
     hipLaunchKernelGGL(k_prep_f64_dot, dim3(cdiv(ns_pad, kBlock)), dim3(kBlock), 0, st, b.src64, q.srcpos, q.zeta,
-                       q.area, q.smap, q.ns, ns_pad);
+                       q.area, q.smap, q.ns, ns_pad, q.ns_dev);
   } else {
     hipLaunchKernelGGL(k_prep_f64_tiles, dim3(sh.ntiles), dim3(kTile), 0, st, b.src64, b.ctr, q.srcpos, q.zeta,
                        q.area, q.smap, q.ns);
@@ -1473,12 +1483,94 @@ BufferPlan plan_buffers(int nt, const Shape& sh) {
 }
 
 // ---------------------------------------------------------------------------
-// per-device workspace for the functor entry points
+// functor entry points: scratch layout, per-device workspace, static scratch
 // ---------------------------------------------------------------------------
+inline size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// Scratch layout of one functor evaluation, as byte offsets from a 256-byte
+// aligned base: source records and split partials planned for nf sources
+// (the active count is known only on the device, after compaction), the
+// compacted source map, self slots and count, and a velocity buffer when the
+// caller asks for the stream function only.
+struct FunctorPlan {
+  Shape shape;
+  size_t src64 = 0, part = 0, smap = 0, self = 0, count = 0, utmp = 0, bytes = 0;
+};
+
+FunctorPlan functor_plan(int64_t nt, int64_t nf, bool need_u) {
+  FunctorPlan p;
+  p.shape = choose_shape((int)nt, (int)std::max<int64_t>(nf, 1), 2, 0);
+  const BufferPlan bp = plan_buffers((int)nt, p.shape);
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    const size_t r = off;
+    off += align256(n);
+    return r;
+  };
+  p.src64 = take(bp.src64);
+  p.part = take(bp.part);
+  p.smap = take(sizeof(int) * (size_t)(nf + 1));
+  p.self = take(sizeof(int) * (size_t)(nf + 1));
+  p.count = take(sizeof(int));
+  p.utmp = need_u ? take(sizeof(double) * 3 * (size_t)nt) : 0;
+  p.bytes = off;
+  return p;
+}
+
+int functor_check(double* u, double* psi, const double* tx, int64_t nt, const double* fx, const double* fzeta,
+                  const double* farea, const uint8_t* fmask, int64_t nf, bool collocated) {
+  if (nt < 0 || nf < 0 || nt > INT32_MAX / 4 || nf > INT32_MAX / 4) return fail(LPM_ERR_ARG, "functor: bad sizes");
+  if (collocated && nt != nf) return fail(LPM_ERR_ARG, "functor: face targets must be the nf faces");
+  if (nt > 0 && (!tx || !fx || !fzeta || !farea || !fmask || (!u && !psi)))
+    return fail(LPM_ERR_ARG, "functor: null pointer");
+  return LPM_OK;
+}
+
+// Queue one functor evaluation on `st` with its scratch at `base`: compact
+// the mask on the device, then one FP64 dot-form evaluation whose kernels read
+// the active count from the device (tiles past it are skipped, so no value
+// crosses to the host and nothing waits).  No active source: every sum is 0.
+int functor_enqueue(const FunctorPlan& fp, char* base, double* u, double* psi, const double* tx, int64_t nt,
+                    const double* fx, const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf,
+                    bool collocated, hipStream_t st) {
+  EvalBuffers b;
+  b.src64 = (Src64*)(base + fp.src64);
+  b.part = (double*)(base + fp.part);
+  int* smap = (int*)(base + fp.smap);
+  int* self = (int*)(base + fp.self);
+  int* count = (int*)(base + fp.count);
+  hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, st, fmask, (int)nf, 0, smap, collocated ? self : nullptr,
+                     count);
+  LPM_HIP(hipGetLastError());
+  EvalRequest q;
+  q.tgt = Pos3{tx, 3, 1};
+  q.nt = (int)nt;
+  q.srcpos = Pos3{fx, 3, 1};
+  q.zeta = fzeta;
+  q.area = farea;
+  q.smap = smap;
+  q.ns = (int)nf;  // upper bound; the kernels take the count from ns_dev
+  q.ns_dev = count;
+  q.self_slot = collocated ? self : nullptr;
+  q.u = MutPos3{u ? u : (double*)(base + fp.utmp), 3, 1};
+  q.psi = psi;
+  q.psi_stride = 1;
+  q.prec = LPM_PREC_FP64;
+  q.form = FORM_DOT;
+  q.want_psi = psi != nullptr;
+  q.shape = fp.shape;
+  return enqueue_eval(q, b, st, nullptr, nullptr);
+}
+
+// Per-device workspace of the allocating functor entry points.  It grows on
+// demand and is shared by every call on the device: each call waits on the
+// device for the previous call's kernels (an event, whatever their stream)
+// before it reuses the workspace.
 struct Workspace {
   std::mutex mu;
   char* base = nullptr;
   size_t bytes = 0;
+  hipEvent_t done = nullptr;  // recorded behind the last call's kernels
 };
 std::mutex g_ws_mu;
 std::map<int, Workspace*> g_ws;
@@ -1490,82 +1582,34 @@ Workspace* workspace_for(int dev) {
   return w;
 }
 
-inline size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
-
-// Functor evaluation on caller-owned device arrays: compact the mask, then
-// one FP64 dot-form evaluation.
+// Functor evaluation on caller-owned device arrays, asynchronous on `stream`.
 int functor_eval(double* u, double* psi, const double* tx, int64_t nt, const double* fx, const double* fzeta,
                  const double* farea, const uint8_t* fmask, int64_t nf, bool collocated, void* stream) {
-  if (nt < 0 || nf < 0 || nt > INT32_MAX / 4 || nf > INT32_MAX / 4) return fail(LPM_ERR_ARG, "functor: bad sizes");
+  int rc = functor_check(u, psi, tx, nt, fx, fzeta, farea, fmask, nf, collocated);
+  if (rc) return rc;
   if (nt == 0) return ok();
-  if (!tx || !fx || !fzeta || !farea || !fmask || (!u && !psi)) return fail(LPM_ERR_ARG, "functor: null pointer");
   int dev = 0;
   LPM_HIP(hipGetDevice(&dev));
   hipStream_t st = (hipStream_t)stream;
   Workspace* ws = workspace_for(dev);
   std::lock_guard<std::mutex> g(ws->mu);
-  // the number of active sources is only known on the device: plan for nf
-  const Shape sh = choose_shape((int)nt, (int)std::max<int64_t>(nf, 1), 2, 0);
-  const BufferPlan bp = plan_buffers((int)nt, sh);
-  const size_t need = align256(bp.src64) + align256(bp.part) + align256(sizeof(int) * (nf + 1)) +
-                      align256(sizeof(int) * (nf + 1)) + align256(sizeof(int)) +
-                      align256(sizeof(double) * 3 * (size_t)nt) + 4096;
-  if (ws->bytes < need) {
+  const FunctorPlan fp = functor_plan(nt, nf, u == nullptr);
+  if (!ws->done) LPM_HIP(hipEventCreateWithFlags(&ws->done, hipEventDisableTiming));
+  if (ws->bytes < fp.bytes) {
     if (ws->base) {
-      LPM_HIP(hipStreamSynchronize(st));
+      // the previous call's kernels still read the old workspace
+      LPM_HIP(hipEventSynchronize(ws->done));
       LPM_HIP(hipFree(ws->base));
       ws->base = nullptr;
       ws->bytes = 0;
     }
-    if (hipMalloc(&ws->base, need) != hipSuccess) return fail(LPM_ERR_NOMEM, "functor: workspace allocation failed");
-    ws->bytes = need;
+    if (hipMalloc(&ws->base, fp.bytes) != hipSuccess) return fail(LPM_ERR_NOMEM, "functor: workspace allocation failed");
+    ws->bytes = fp.bytes;
+  } else {
+    LPM_HIP(hipStreamWaitEvent(st, ws->done, 0));
   }
-  char* p = ws->base;
-  auto carve = [&](size_t n) {
-    char* r = p;
-    p += align256(n);
-    return r;
-  };
-  EvalBuffers b;
-  b.src64 = (Src64*)carve(bp.src64);
-  b.part = (double*)carve(bp.part);
-  int* smap = (int*)carve(sizeof(int) * (nf + 1));
-  int* self = (int*)carve(sizeof(int) * (nf + 1));
-  int* count = (int*)carve(sizeof(int));
-  double* utmp = (double*)carve(sizeof(double) * 3 * (size_t)nt);
-  hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, st, fmask, (int)nf, 0, smap, collocated ? self : nullptr,
-                     count);
-  LPM_HIP(hipGetLastError());
-  int ns = 0;
-  LPM_HIP(hipMemcpyAsync(&ns, count, sizeof(int), hipMemcpyDeviceToHost, st));
-  LPM_HIP(hipStreamSynchronize(st));
-  EvalRequest q;
-  q.tgt = Pos3{tx, 3, 1};
-  q.nt = (int)nt;
-  q.srcpos = Pos3{fx, 3, 1};
-  q.zeta = fzeta;
-  q.area = farea;
-  q.smap = smap;
-  q.ns = ns;
-  q.self_slot = collocated ? self : nullptr;
-  q.u = MutPos3{u ? u : utmp, 3, 1};
-  q.psi = psi;
-  q.psi_stride = 1;
-  q.prec = LPM_PREC_FP64;
-  q.form = FORM_DOT;
-  q.want_psi = psi != nullptr;
-  q.shape = choose_shape((int)nt, std::max(ns, 1), 2, 0);
-  if (q.shape.splits > sh.splits || q.shape.ntiles > sh.ntiles) q.shape = sh;
-  if (ns == 0) {
-    // no active source: every sum is zero
-    if (u) LPM_HIP(hipMemsetAsync(u, 0, sizeof(double) * 3 * nt, st));
-    if (psi) LPM_HIP(hipMemsetAsync(psi, 0, sizeof(double) * nt, st));
-    return ok();
-  }
-  int rc = enqueue_eval(q, b, st, nullptr, nullptr);
-  if (rc) return rc;
-  // the workspace is reused by the next call: finish before releasing it
-  LPM_HIP(hipStreamSynchronize(st));
+  if ((rc = functor_enqueue(fp, ws->base, u, psi, tx, nt, fx, fzeta, farea, fmask, nf, collocated, st))) return rc;
+  LPM_HIP(hipEventRecord(ws->done, st));
   return ok();
 }
 
@@ -1584,6 +1628,7 @@ struct lpm_bve_ctx {
   // position and vorticity (divided faces are not evaluated and keep theirs)
   std::vector<int64_t> fmap;
   std::vector<double> face_x_host, face_zeta_host;
+  std::vector<uint8_t> fmask_host;  // every tree face's mask (state set from arrays)
   // device arrays (SoA, target-indexed: vertices then tree faces)
   double *x = nullptr, *y = nullptr, *z = nullptr, *zeta = nullptr;
   double *u = nullptr, *v = nullptr, *w = nullptr, *psi = nullptr;
@@ -2067,53 +2112,71 @@ int lpm_bve_destroy(lpm_bve_ctx* c) {
 int lpm_bve_set_state(lpm_bve_ctx* c, int64_t nv, const double* vx, const double* vzeta, int64_t nf,
                       const double* fx, const double* fzeta, const double* farea, const uint8_t* fmask) {
   if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_state: null context");
-  if (nv < 0 || nf < 1 || nv + nf > (int64_t)INT32_MAX / 8)
+  if (nv < 0 || nf < 1 || nv > (int64_t)INT32_MAX || nf > (int64_t)INT32_MAX)
     return fail(LPM_ERR_ARG, "lpm_bve_set_state: sizes out of range");
   if ((nv && (!vx || !vzeta)) || !fx || !fzeta || !farea || !fmask)
     return fail(LPM_ERR_ARG, "lpm_bve_set_state: null array");
-  LPM_HIP(hipSetDevice(c->device));
-  LPM_HIP(hipStreamSynchronize(c->stream));
-  free_ctx_memory(c);
-  c->eb = EvalBuffers();
-  c->nv = nv;
-  c->nf = nf;
   // Divided faces are sources of nothing (mask) and, as targets, sit on
   // their centre child's centroid, where the reference's collocated sum
   // divides by 1 - x.y = 0; they are kept on the host and not evaluated.
-  c->fmap.clear();
+  std::vector<int64_t> fmap;
   for (int64_t f = 0; f < nf; ++f)
-    if (!fmask[f]) c->fmap.push_back(f);
-  c->ns = (int)c->fmap.size();
-  if (c->ns == 0) return fail(LPM_ERR_ARG, "lpm_bve_set_state: no active (unmasked) face");
+    if (!fmask[f]) fmap.push_back(f);
+  const int64_t ns = (int64_t)fmap.size();
+  if (ns == 0) return fail(LPM_ERR_ARG, "lpm_bve_set_state: no active (unmasked) face");
+  if (nv + ns > (int64_t)INT32_MAX / 8)
+    return fail(LPM_ERR_ARG, "lpm_bve_set_state: vertices + active faces out of range (the limit of lpm_bve_state_bytes)");
+  LPM_HIP(hipSetDevice(c->device));
+  LPM_HIP(hipStreamSynchronize(c->stream));
+  // Same sizes and mask as the resident state (the reference-side copy-in of
+  // every advance_timestep): update the arrays in place, keeping every buffer
+  // and captured step graph, with no allocation and no free.
+  const bool in_place = c->x && !c->mesh && c->nv == nv && c->nf == nf && (int64_t)c->fmask_host.size() == nf &&
+                        std::equal(c->fmask_host.begin(), c->fmask_host.end(), fmask);
+  if (!in_place) {
+    free_ctx_memory(c);
+    c->eb = EvalBuffers();
+    c->nv = nv;
+    c->nf = nf;
+    c->fmap = std::move(fmap);
+    c->ns = (int)ns;
+    c->nt = (int)(nv + ns);
+    c->fmask_host.assign(fmask, fmask + nf);
+  }
   c->face_x_host.assign(fx, fx + 3 * nf);
   c->face_zeta_host.assign(fzeta, fzeta + nf);
-  c->nt = (int)(nv + c->ns);
   const int nt = c->nt;
   // host staging: SoA, target-indexed (vertices, then active faces in tree order)
-  std::vector<double> h(4 * (size_t)nt), harea(nt, 0.0);
-  std::vector<int> hsmap(c->ns), hself(nt, -1);
+  std::vector<double> h(5 * (size_t)nt);
+  double* harea = h.data() + 4 * (size_t)nt;
   for (int64_t i = 0; i < nv; ++i) {
     for (int k = 0; k < 3; ++k) h[k * (size_t)nt + i] = vx[3 * i + k];
     h[3 * (size_t)nt + i] = vzeta[i];
+    harea[i] = 0.0;
   }
   for (int j = 0; j < c->ns; ++j) {
     const int64_t f = c->fmap[j], i = nv + j;
     for (int k = 0; k < 3; ++k) h[k * (size_t)nt + i] = fx[3 * f + k];
     h[3 * (size_t)nt + i] = fzeta[f];
     harea[i] = farea[f];
-    hsmap[j] = (int)i;
-    hself[i] = j;
   }
-  int rc = ctx_alloc_state(c, true);
-  if (rc) return rc;
   const size_t d = sizeof(double) * (size_t)nt;
-  LPM_HIP(hipMemcpyAsync(c->x, h.data(), d, hipMemcpyHostToDevice, c->stream));
-  LPM_HIP(hipMemcpyAsync(c->y, h.data() + nt, d, hipMemcpyHostToDevice, c->stream));
-  LPM_HIP(hipMemcpyAsync(c->z, h.data() + 2 * (size_t)nt, d, hipMemcpyHostToDevice, c->stream));
+  if (!in_place) {
+    std::vector<int> hsmap(c->ns), hself(nt, -1);
+    for (int j = 0; j < c->ns; ++j) {
+      hsmap[j] = (int)(nv + j);
+      hself[nv + j] = j;
+    }
+    int rc = ctx_alloc_state(c, true);
+    if (rc) return rc;
+    LPM_HIP(hipMemcpyAsync(c->smap, hsmap.data(), sizeof(int) * c->ns, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(hipMemcpyAsync(c->self_slot, hself.data(), sizeof(int) * nt, hipMemcpyHostToDevice, c->stream));
+    LPM_HIP(hipStreamSynchronize(c->stream));
+  }
+  // positions (one [3][nt] block), vorticity, areas
+  LPM_HIP(hipMemcpyAsync(c->x, h.data(), 3 * d, hipMemcpyHostToDevice, c->stream));
   LPM_HIP(hipMemcpyAsync(c->zeta, h.data() + 3 * (size_t)nt, d, hipMemcpyHostToDevice, c->stream));
-  LPM_HIP(hipMemcpyAsync(c->area, harea.data(), d, hipMemcpyHostToDevice, c->stream));
-  LPM_HIP(hipMemcpyAsync(c->smap, hsmap.data(), sizeof(int) * c->ns, hipMemcpyHostToDevice, c->stream));
-  LPM_HIP(hipMemcpyAsync(c->self_slot, hself.data(), sizeof(int) * nt, hipMemcpyHostToDevice, c->stream));
+  LPM_HIP(hipMemcpyAsync(c->area, harea, d, hipMemcpyHostToDevice, c->stream));
   LPM_HIP(hipStreamSynchronize(c->stream));
   c->vel_valid = false;
   c->psi_valid = false;
@@ -2139,6 +2202,7 @@ int lpm_bve_set_state_level(lpm_bve_ctx* c, int level, const lpm_ic_params* ic) 
   c->ns = m->ns;
   c->nt = m->nt;
   c->fmap.clear();
+  c->fmask_host.clear();
   c->face_x_host.clear();
   c->face_zeta_host.clear();
   if ((rc = ctx_alloc_state(c, false))) return rc;
@@ -2153,12 +2217,12 @@ int lpm_bve_set_state_level(lpm_bve_ctx* c, int level, const lpm_ic_params* ic) 
   for (int k = 0; k < nv; ++k)
     hv[k] = make_double4(ic->centers[3 * k], ic->centers[3 * k + 1], ic->centers[3 * k + 2], ic->amps[k]);
   double4* dv = nullptr;
-  double* scratch = nullptr;
+  double* ic_scratch = nullptr;
   if (hipMalloc(&dv, sizeof(double4) * hv.size()) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
   c->allocations.push_back(dv);
   const size_t scratch_bytes = sizeof(double2) * dev::kMomentBlocks + 256;
-  if (hipMalloc(&scratch, scratch_bytes) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
-  c->allocations.push_back(scratch);
+  if (hipMalloc(&ic_scratch, scratch_bytes) != hipSuccess) return fail(LPM_ERR_NOMEM, "device allocation failed");
+  c->allocations.push_back(ic_scratch);
   LPM_HIP(hipMemcpyAsync(dv, hv.data(), sizeof(double4) * hv.size(), hipMemcpyHostToDevice, c->stream));
   IcArgs a;
   a.sbr = ic->sbr;
@@ -2170,8 +2234,8 @@ int lpm_bve_set_state_level(lpm_bve_ctx* c, int level, const lpm_ic_params* ic) 
   a.s2 = ic->sigma * ic->sigma;
   const dim3 grid(cdiv(nt, kBlock));
   hipLaunchKernelGGL(k_ic_eval, grid, dim3(kBlock), 0, c->stream, Pos3{c->x, 1, (long)nt}, nt, a, c->zeta);
-  double* mean = scratch;
-  double2* part = reinterpret_cast<double2*>(scratch + 32);
+  double* mean = ic_scratch;
+  double2* part = reinterpret_cast<double2*>(ic_scratch + 32);
   c->ic_centered = ic->subtract_mean != 0;
   if (c->ic_centered) {
     hipLaunchKernelGGL(k_zeta_moments, dim3(kMomentBlocks), dim3(kBlock), 0, c->stream, c->zeta, c->area, nt, part);
@@ -2615,6 +2679,10 @@ struct lpm_bve_ensemble {
   // plan (rebuilt when membership, generations or the step kind change)
   bool plan_valid = false;
   std::vector<uint64_t> plan_generations;
+  // each member's launch shape when the plan was built (R, splits, tiles per
+  // split, near factor): a change (lpm_bve_set_launch, lpm_bve_set_near_factor,
+  // a new autotuned rule) rebuilds the plan and its graphs
+  std::vector<std::tuple<int, int, int, float>> plan_shapes;
   int plan_prec = -1, plan_form = -1;
   std::vector<int> plan_R;                 // distinct R values, one launch each
   std::vector<int> plan_nitems;            // items per R launch
@@ -2659,11 +2727,17 @@ void ens_free_buffers(lpm_bve_ensemble* e) {
   e->d_counters = nullptr;
 }
 
+std::tuple<int, int, int, float> shape_key(const lpm_bve_ctx* c) {
+  const Shape sh = ctx_shape(c);
+  return std::make_tuple(sh.R, sh.splits, sh.tiles_per_split, sh.near_factor);
+}
+
 bool ens_plan_current(const lpm_bve_ensemble* e, int prec, int form) {
   if (!e->plan_valid || e->plan_prec != prec || e->plan_form != form) return false;
-  if (e->plan_generations.size() != e->members.size()) return false;
+  if (e->plan_generations.size() != e->members.size() || e->plan_shapes.size() != e->members.size()) return false;
   for (size_t m = 0; m < e->members.size(); ++m)
-    if (e->plan_generations[m] != e->members[m]->generation) return false;
+    if (e->plan_generations[m] != e->members[m]->generation || e->plan_shapes[m] != shape_key(e->members[m]))
+      return false;
   return true;
 }
 
@@ -2721,12 +2795,15 @@ int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
   LPM_HIP(hipMemcpyAsync(e->d_mem_w, hw.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice, e->stream));
   LPM_HIP(hipMemcpyAsync(e->d_mem_x, hx.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice, e->stream));
   size_t k = 0;
+  // one host list per launch, all alive until the uploads below have run (a
+  // copy from pageable memory may still be queued behind the running step)
+  std::vector<std::vector<EnsItem>> lists(byR.size());
   for (auto& kv : byR) {
     auto& v = kv.second;
     std::stable_sort(v.begin(), v.end(), [](const std::pair<long, EnsItem>& a, const std::pair<long, EnsItem>& b) {
       return a.first > b.first;
     });
-    std::vector<EnsItem> items;
+    std::vector<EnsItem>& items = lists[k];
     items.reserve(v.size());
     for (auto& pr : v) items.push_back(pr.second);
     if (k >= e->item_bufs.size()) {
@@ -2749,7 +2826,11 @@ int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
   // the host copies above must outlive the uploads
   LPM_HIP(hipStreamSynchronize(e->stream));
   e->plan_generations.clear();
-  for (auto* c : e->members) e->plan_generations.push_back(c->generation);
+  e->plan_shapes.clear();
+  for (auto* c : e->members) {
+    e->plan_generations.push_back(c->generation);
+    e->plan_shapes.push_back(shape_key(c));
+  }
   e->plan_prec = prec;
   e->plan_form = form;
   e->plan_valid = true;

@@ -1,6 +1,7 @@
 // Mesh and field files; see lpm_meshio.hpp.
 #include "lpm_meshio.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -9,7 +10,7 @@
 
 namespace lpm {
 
-void add_mesh_vars(nc::Writer& w, const TreeMesh& m, const double* phys_v, const double* phys_f) {
+void add_mesh_vars(nc::Writer& w, const TreeMesh& m, const double* phys_v, const double* phys_f, const double* area) {
   const int ndim = w.dim("ndim", 3);
   const int nverts = w.dim("nverts", m.nverts());
   const int nedges = w.dim("nedges", m.n_edges);
@@ -33,7 +34,7 @@ void add_mesh_vars(nc::Writer& w, const TreeMesh& m, const double* phys_v, const
   std::iota(centers.begin(), centers.end(), 0);
   w.var("phys_crds_faces", nc::kDouble, {nfaces, ndim}, phys_f ? phys_f : m.face_phys.data()->v);
   w.var("lag_crds_faces", nc::kDouble, {nfaces, ndim}, m.face_lag.data()->v);
-  w.var("face_area", nc::kDouble, {nfaces}, m.face_area.data());
+  w.var("face_area", nc::kDouble, {nfaces}, area ? area : m.face_area.data());
   w.var("face_mask", nc::kByte, {nfaces}, m.face_mask.data());
   w.var("face_edges", nc::kInt, {nfaces, nfaceverts}, m.face_edges.data());
   w.var("face_verts", nc::kInt, {nfaces, nfaceverts}, m.face_verts.data());
@@ -52,9 +53,16 @@ bool write_bve_nc(const std::string& path, const BveFields& s, const TreeMesh* m
     err = "mesh sizes differ from the state's";
     return false;
   }
+  // the tree's topology describes the state only if the state's divided
+  // faces are the tree's; areas are the state's own (a caller may have set
+  // areas other than the tree's, and a restart must read them back)
+  if (mesh && !std::equal(s.fmask.begin(), s.fmask.end(), mesh->face_mask.begin())) {
+    err = "the state's face mask differs from the tree's: write it without a tree level";
+    return false;
+  }
   nc::Writer w;
   if (mesh) {
-    add_mesh_vars(w, *mesh, s.vx.data(), s.fx.data());
+    add_mesh_vars(w, *mesh, s.vx.data(), s.fx.data(), s.farea.data());
   } else {
     const int ndim = w.dim("ndim", 3);
     const int nverts = w.dim("nverts", s.nv);

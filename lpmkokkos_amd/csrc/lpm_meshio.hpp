@@ -24,7 +24,10 @@ namespace lpm {
 // `w`.  phys_v [nverts][3] / phys_f [nfaces][3] replace the mesh's physical
 // coordinates when non-null (the particles' current positions).  Pointers
 // into `m` and the arguments must stay valid until w.write().
-void add_mesh_vars(nc::Writer& w, const TreeMesh& m, const double* phys_v, const double* phys_f);
+// phys_v, phys_f, area (may be null: the tree's own) replace the tree's
+// physical coordinates and face areas.
+void add_mesh_vars(nc::Writer& w, const TreeMesh& m, const double* phys_v, const double* phys_f,
+                   const double* area = nullptr);
 
 // A BVE state on the host, faces in tree order (divided faces included).
 struct BveFields {

@@ -291,6 +291,9 @@ class Server:
         for r in requests:
             if request_bytes(r.spec) > self.budget:
                 raise ValueError(f"request {r.rid} (level {r.spec.level}) exceeds the memory budget")
+            if r.spec.level >= self.large_level and not self.lane_streams:
+                raise ValueError(f"request {r.rid} (level {r.spec.level}) needs a large lane and the server has none "
+                                 f"(large_lanes=0, large_level={self.large_level})")
         self.warm(r.spec.level for r in requests)
         pending = sorted(requests, key=lambda r: r.arrival)
         done: list[Request] = []
@@ -395,6 +398,9 @@ class Server:
             "latency_p50_s_below_large": float(np.percentile(below, 50)),
             "latency_p50_s_by_level": {int(l): float(np.percentile([r.latency for r in done if r.spec.level == l], 50))
                                        for l in sorted({r.spec.level for r in done})},
+            "latency_p99_s_by_level": {int(l): float(np.percentile([r.latency for r in done if r.spec.level == l], 99))
+                                       for l in sorted({r.spec.level for r in done})},
+            "latency_max_s": float(lat.max()),
             "admission_mean_s": float(np.mean([r.admit_s for r in done] or [0.0])),
             "admission_max_s": float(np.max([r.admit_s for r in done] or [0.0])),
             "levels": {int(l): int(sum(1 for r in done if r.spec.level == l)) for l in sorted({r.spec.level for r in done})},

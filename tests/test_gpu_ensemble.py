@@ -114,3 +114,79 @@ def test_reference_scheme_members_match_alone():
     ens.close()
     for c in alone + mixed:
         c.close()
+
+
+def test_shape_change_rebuilds_plan():
+    """A member's launch shape changed between ensemble steps (fewer source
+    splits, another near factor) rebuilds the work list and the step graph:
+    each member still equals the same member run alone with the same changes,
+    bit for bit, with and without graph replay."""
+    sp = specs()[:4]
+    for use_graph in (False, True):
+        alone = [E.make_member(s) for s in sp]
+        mixed = [E.make_member(s) for s in sp]
+        ens = E.Ensemble(0)
+        for c in mixed:
+            ens.add(c)
+        for c in alone:
+            c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=2, use_graph=use_graph)
+        ens.advance(0.005, TWO_PI, nsteps=2, use_graph=use_graph)
+        for c in (alone[2], mixed[2]):
+            c.set_launch(1, 1)
+            c.set_near_factor(6.0)
+        for c in alone:
+            c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=2, use_graph=use_graph)
+        ens.advance(0.005, TWO_PI, nsteps=2, use_graph=use_graph)
+        for i, (a, b) in enumerate(zip(alone, mixed)):
+            assert same(a, b), (use_graph, i)
+        ens.close()
+        for c in alone + mixed:
+            c.close()
+
+
+@pytest.mark.timeout(400)
+def test_batching_configuration_all_members(omesh):
+    """The batching configuration in full: 64 members with levels drawn
+    uniformly from 3..7 (seed 3), RH wave with K x U(0.95, 1.05) plus 8
+    Gaussian vortices, 10 FP32C Lagrangian steps of dt = 0.005 in one
+    ensemble (persistent pair kernel per evaluation, graph-replayed steps).
+    Every one of the 64 members equals the same member advanced alone, bit
+    for bit; for one member per level the end state's FP64 Lagrangian
+    velocity matches the oracle's chord-form sums (src/LpmBVEKernels.hpp:36-44
+    with |y - x|^2 / 2) on a strided subset of targets."""
+    from oracle import oracle as O
+    from helpers import normwise
+    sp = E.ensemble_config(seed=3, n=64)
+    assert sorted({s.level for s in sp}) == [3, 4, 5, 6, 7]
+    mixed = [E.make_member(s) for s in sp]
+    ens = E.Ensemble(0)
+    for c in mixed:
+        ens.add(c)
+    ens.advance(0.005, TWO_PI, nsteps=10)
+    ens.synchronize()
+    for s, b in zip(sp, mixed):
+        a = E.make_member(s)
+        a.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=10)
+        assert same(a, b), s
+        a.close()
+    ens.close()
+    checked = set()
+    for s, c in zip(sp, mixed):
+        if s.level in checked:
+            continue
+        checked.add(s.level)
+        c.solve(L.PREC_FP64, L.SCHEME_LAGR)
+        st = c.get_state()
+        m = omesh(s.level)
+        lv = m.fmask == 0
+        stride = max(1, m.nv // 300)
+        ov = O.velocity_chord(st["vx"], st["fx"], st["fzeta"], m.farea, m.fmask, False, stride=stride)
+        assert normwise(st["vu"][::stride], ov[::stride]) < 1e-12, s.level
+        of = O.velocity_chord(st["fx"], st["fx"], st["fzeta"], m.farea, m.fmask, True, stride=stride)
+        rows = np.zeros(m.nf, bool)
+        rows[::stride] = True
+        rows &= lv
+        assert normwise(st["fu"][rows], of[rows]) < 1e-12, s.level
+    assert checked == {3, 4, 5, 6, 7}
+    for c in mixed:
+        c.close()

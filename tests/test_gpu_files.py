@@ -149,6 +149,40 @@ def test_checkpoint_without_tree(tmp_path):
     b.close()
 
 
+def test_checkpoint_keeps_the_state_areas(tmp_path):
+    """A state whose areas differ from the tree's (a caller's mesh) is
+    checkpointed with the tree level named: the file holds the state's areas,
+    and the resumed run is bit-identical to the one that never stopped.  A
+    mask that differs from the tree's is refused with a tree level."""
+    level = 3
+    m = L.TreeMesh(level)
+    rng = np.random.default_rng(11)
+    area = m.farea * (1 + 1e-3 * rng.uniform(-1, 1, m.nfaces))
+    ic = field()
+    a = L.Context(0)
+    a.set_state(m.vx, ic.eval_many(m.vx), m.fx, ic.eval_many(m.fx), area, m.fmask)
+    a.advance(DT, TWO_PI, L.SCHEME_LAGR, L.PREC_FP64, nsteps=2)
+    path = str(tmp_path / "areas.nc")
+    a.checkpoint(path, time=2 * DT, omega=TWO_PI, level=level)
+    f = scipy.io.netcdf_file(path, "r", mmap=False)
+    lv = m.fmask == 0
+    assert np.array_equal(np.array(f.variables["face_area"][:])[lv], area[lv])
+    a.advance(DT, TWO_PI, L.SCHEME_LAGR, L.PREC_FP64, nsteps=2)
+    b = L.Context(0)
+    b.restore(path)
+    b.advance(DT, TWO_PI, L.SCHEME_LAGR, L.PREC_FP64, nsteps=2)
+    states_equal(a.get_state(), b.get_state())
+    mask = m.fmask.copy()
+    mask[np.nonzero(lv)[0][:4]] = 1  # four leaves masked: not the tree's mask
+    c = L.Context(0)
+    c.set_state(m.vx, ic.eval_many(m.vx), m.fx, ic.eval_many(m.fx), area, mask)
+    with pytest.raises(L.LpmError):
+        c.checkpoint(str(tmp_path / "mask.nc"), omega=TWO_PI, level=level)
+    c.checkpoint(str(tmp_path / "mask.nc"), omega=TWO_PI)  # without the tree: particles only
+    for x in (a, b, c):
+        x.close()
+
+
 # --- VTK readers -------------------------------------------------------------
 
 class _Cursor:

@@ -145,6 +145,40 @@ def test_level4_rh_wave_100_steps_ref(omesh):
     c.close()
 
 
+def test_copy_in_copy_out_binding_matches_resident(omesh):
+    """The reference-side binding (every advance_timestep copies the caller's
+    arrays in with lpm_bve_set_state + lpm_bve_set_velocity, steps once, and
+    copies them out with lpm_bve_get_state; src/LpmBVERK4_Impl.hpp:42-59)
+    over the level-4 Rossby-Haurwitz configuration for 100 reference-scheme
+    steps equals a resident run of lpm_bve_advance, bit for bit.  The tree
+    faces (divided ones included) go in and out as the reference holds them;
+    set_state updates the resident buffers in place."""
+    import time
+    m = omesh(4)
+    vz, fz = rh_field(m.vx), rh_field(m.fx)
+    res = L.Context(0)
+    res.set_state(m.vx, vz, m.fx, fz, m.farea, m.fmask)
+    res.solve(L.PREC_FP64, L.SCHEME_REF)
+    s = res.get_state()
+    vx, vzeta, vu, fx, fzeta, fu = (s[k].copy() for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"))
+    res.advance(0.01, TWO_PI, L.SCHEME_REF, L.PREC_FP64, nsteps=100)
+    c = L.Context(0)
+    t0 = time.perf_counter()
+    for _ in range(100):
+        c.set_state(vx, vzeta, fx, fzeta, m.farea, m.fmask)
+        c.set_velocity(vu, fu)
+        c.advance(0.01, TWO_PI, L.SCHEME_REF, L.PREC_FP64, nsteps=1)
+        s = c.get_state()
+        vx, vzeta, vu, fx, fzeta, fu = (s[k] for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"))
+    dt = time.perf_counter() - t0
+    r = res.get_state()
+    for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
+        assert np.array_equal(r[k], s[k]), k
+    print(f"copy-in/copy-out binding: {1e3 * dt / 100:.2f} ms per step at level 4")
+    res.close()
+    c.close()
+
+
 def test_lagr_scheme_fp64(omesh):
     m = omesh(3)
     lm = LeafMesh(m)

@@ -128,3 +128,35 @@ def test_size_class_lanes_step_independently():
         for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
             assert np.array_equal(s[k], r.result[k]), (r.rid, k)
         c.close()
+
+
+@pytest.mark.timeout(300)
+def test_level8_request_in_a_large_lane_beside_batched_requests():
+    """The serving configuration's largest size: a level-8 request (1.31M
+    sources) runs in a large lane while requests of levels 4-7 are batched in
+    their size-class lanes; every result, the level-8 one included, equals
+    the request run alone, bit for bit."""
+    reqs = [S.Request(rid=0, arrival=0.0, spec=E.MemberSpec(level=8, rh_n=3, nvort=6, vortex_seed=80), steps=2)]
+    reqs += [S.Request(rid=1 + k, arrival=0.05 * k, spec=E.MemberSpec(level=lev, rh_n=2 + k % 5, nvort=4 + k,
+                                                                     vortex_seed=90 + k), steps=3)
+             for k, lev in enumerate([4, 5, 6, 7, 4, 5])]
+    srv = S.Server(keep_results=True)  # default lanes: levels <= 5, 6, 7 batched; level 8 large
+    res = srv.run(reqs)
+    srv.close()
+    assert res["requests"] == 7 and res["large_lane_requests"] == 1
+    assert res["levels"] == {4: 2, 5: 2, 6: 1, 7: 1, 8: 1}
+    for r in reqs:
+        c = E.make_member(r.spec)
+        c.advance(0.005, 2 * math.pi, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=r.steps)
+        s = c.get_state()
+        for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
+            assert np.array_equal(s[k], r.result[k]), (r.rid, k)
+        c.close()
+
+
+def test_large_request_without_large_lanes_is_refused():
+    reqs = [S.Request(rid=0, arrival=0.0, spec=E.MemberSpec(level=4, vortex_seed=1), steps=1)]
+    srv = S.Server(large_level=4, large_lanes=0)
+    with pytest.raises(ValueError):
+        srv.run(reqs)
+    srv.close()
