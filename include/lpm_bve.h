@@ -186,6 +186,25 @@ int lpm_bve_vertex_solve(double* psi, double* u, const double* vx, int64_t nv, c
 int lpm_bve_face_solve(double* psi, double* u, const double* fx, const double* fzeta, const double* farea,
                        const uint8_t* fmask, int64_t nf, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The functor evaluation above (psi and/or u, either may
+ * be NULL but not both; collocated != 0: the targets are the nf faces fx
+ * themselves, each face's own source skipped, as lpm_bve_face_solve; else
+ * nt targets tx, as lpm_bve_vertex_solve), with the same kernels and launch
+ * shape, so the results are bit-identical to those entry points'.  Its
+ * scratch (source records, split partials, source map) lives in a fixed
+ * static device array of LPM_FUNCTOR_STATIC_SCRATCH_BYTES that each device
+ * holds until the process ends; it calls no hipMalloc or hipFree and never
+ * waits.  LPM_ERR_ARG when the sizes need more scratch than that (the
+ * evaluations of trees up to level 6, 109,220 faces, fit).  Every call on a
+ * device shares that device's array: calls that overlap on one device race
+ * (queue them on one stream). */
+#define LPM_FUNCTOR_STATIC_SCRATCH_BYTES (64ull << 20)
+int lpm_bve_functor_static(double* psi, double* u, const double* tx, int64_t nt, const double* fx,
+                           const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, int collocated,
+                           void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Solver context: the BVESphere fields and BVERK4 stage buffers of    */
 /* one problem, resident in device memory (src/LpmBVESphere.hpp:18-80, */

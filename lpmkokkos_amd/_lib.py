@@ -67,6 +67,7 @@ SIGNATURES = {
     "lpm_bve_face_streamfn": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
     "lpm_bve_vertex_solve": [_dpn, _dpn, _dpn, C.c_int64, _dpn, _dpn, _dpn, _dpn, C.c_int64, C.c_void_p],
     "lpm_bve_face_solve": [_dpn, _dpn, _dpn, _dpn, _dpn, _dpn, C.c_int64, C.c_void_p],
+    "lpm_bve_functor_static": [_dpn, _dpn, _dpn, C.c_int64, _dpn, _dpn, _dpn, _dpn, C.c_int64, C.c_int, C.c_void_p],
     "lpm_bve_create": [C.c_int, C.POINTER(C.c_void_p)],
     "lpm_bve_create_on_stream": [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)],
     "lpm_bve_destroy": [C.c_void_p],

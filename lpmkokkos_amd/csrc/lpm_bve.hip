@@ -1613,6 +1613,43 @@ int functor_eval(double* u, double* psi, const double* tx, int64_t nt, const dou
   return ok();
 }
 
+// Per-device address of the static functor scratch (`scratch` below), looked
+// up once per device with that device current.
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch_ptr;
+
+int static_scratch_for(int dev, char** out);
+
+}  // namespace
+}  // namespace lpm
+
+// Device scratch of lpm_bve_functor_static: a fixed array in every device's
+// copy of the code object, held until the process ends.  Host code never
+// passes this name to a kernel or a copy (on the host it is only a stand-in
+// for the symbol): it uses the address hipGetSymbolAddress returns for the
+// current device.
+constexpr size_t kStaticScratchBytes = LPM_FUNCTOR_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace lpm {
+namespace {
+
+int static_scratch_for(int dev, char** out) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return LPM_OK;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(LPM_ERR_HIP, std::string("lpm_bve_functor_static: scratch symbol address: ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = (char*)p;
+  *out = (char*)p;
+  return LPM_OK;
+}
+
 }  // namespace
 }  // namespace lpm
 
@@ -2067,6 +2104,27 @@ int lpm_bve_face_solve(double* psi, double* u, const double* fx, const double* f
                        const uint8_t* fmask, int64_t nf, void* stream) {
   if (!psi || !u) return fail(LPM_ERR_ARG, "lpm_bve_face_solve: null output");
   return functor_eval(u, psi, fx, nf, fx, fzeta, farea, fmask, nf, true, stream);
+}
+
+int lpm_bve_functor_static(double* psi, double* u, const double* tx, int64_t nt, const double* fx,
+                           const double* fzeta, const double* farea, const uint8_t* fmask, int64_t nf, int collocated,
+                           void* stream) {
+  const bool col = collocated != 0;
+  int rc = functor_check(u, psi, tx, nt, fx, fzeta, farea, fmask, nf, col);
+  if (rc) return rc;
+  if (nt == 0) return ok();
+  const FunctorPlan fp = functor_plan(nt, nf, u == nullptr);
+  if (fp.bytes > kStaticScratchBytes)
+    return fail(LPM_ERR_ARG, "lpm_bve_functor_static: needs " + std::to_string(fp.bytes) +
+                                 " bytes of scratch for these sizes, above its fixed " +
+                                 std::to_string(kStaticScratchBytes) + " (use the allocating functor entry points)");
+  int dev = 0;
+  LPM_HIP(hipGetDevice(&dev));
+  char* base = nullptr;
+  if ((rc = static_scratch_for(dev, &base))) return rc;
+  if ((rc = functor_enqueue(fp, base, u, psi, tx, nt, fx, fzeta, farea, fmask, nf, col, (hipStream_t)stream)))
+    return rc;
+  return ok();
 }
 
 int lpm_bve_create(int device, lpm_bve_ctx** out) {
