@@ -68,8 +68,19 @@ int lpm_device_count(int* count);
 /* Mesh build (host).                                                  */
 /* ------------------------------------------------------------------ */
 
-/* MeshSeed<IcosTriSphereSeed>::setMaxAllocations (src/LpmMeshSeed.cpp:296-305):
- * vertex, edge and tree-face counts of a level-`level` icosahedral tree. */
+/* The spherical mesh seeds (src/LpmMeshSeed.hpp:111-147): the icosahedral
+ * triangle seed and the cubed-sphere quadrilateral seed (4 vertices per
+ * face; a divided face's centre becomes its children's shared corner,
+ * src/LpmFaces.cpp:251-381). */
+typedef enum {
+  LPM_SEED_ICOS_TRI = 0,     /* IcosTriSphereSeed, mesh_seeds/icosTriSphereSeed.dat */
+  LPM_SEED_CUBED_SPHERE = 1  /* CubedSphereSeed, mesh_seeds/cubedSphereSeed.dat */
+} lpm_seed_kind;
+
+/* MeshSeed<Seed>::setMaxAllocations (src/LpmMeshSeed.cpp:296-305, counts of
+ * :369-391): vertex, edge and tree-face counts of a level-`level` tree. */
+int lpm_seed_alloc(int seed_kind, int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces);
+/* The same for the icosahedral seed. */
 int lpm_icos_alloc(int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces);
 
 /* PolyMesh2d<IcosTriSphereSeed>::treeInit (src/LpmPolyMesh2d.cpp:30-44) with
@@ -80,6 +91,12 @@ int lpm_icos_alloc(int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces)
  * counts (non-NULL) receives {nverts, nedges, nfaces, nleaves}. */
 int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vlag, double* fx, double* flag,
                        double* farea, uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts);
+/* PolyMesh2d<Seed>::treeInit for either seed (the seed file, if given, in the
+ * reference's format for that seed); fverts is [nfaces][3] for the
+ * icosahedral seed and [nfaces][4] for the cubed sphere. */
+int lpm_seed_tree_init(int seed_kind, int level, const char* seed_path, double* vx, double* vlag, double* fx,
+                       double* flag, double* farea, uint8_t* fmask, int32_t* fverts, int32_t* fparent,
+                       int64_t* counts);
 
 /* NcWriter::writePolymesh (src/LpmNetCDF_Impl.hpp:46-218) of the treeInit mesh
  * (as tests/LpmNetCDFTest.cpp:95-104 writes icostri_sphere.nc): dimensions
@@ -87,17 +104,22 @@ int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vla
  * face topology and tree, areas and mask; attributes MeshSeed, FaceKind, Geom,
  * baseTreeDepth.  A netCDF classic (64-bit offset) file. */
 int lpm_mesh_write_nc(int level, const char* seed_path, const char* path);
+/* The same for either seed (nfaceverts 4, MeshSeed "CubedSphereSeed",
+ * FaceKind "quad" for the cubed sphere). */
+int lpm_seed_mesh_write_nc(int seed_kind, int level, const char* seed_path, const char* path);
 
 /* PolyMesh2d::outputVtk (src/LpmPolyMesh2d.cpp:79-85) of the treeInit mesh:
  * vertices, leaf faces and their areas ("area" cell data), in one of the
  * lpm_vtk_format layouts. */
 int lpm_mesh_write_vtk(int level, const char* seed_path, const char* path, int format);
+int lpm_seed_mesh_write_vtk(int seed_kind, int level, const char* seed_path, const char* path, int format);
 
 /* PolyMeshReader (src/LpmNetCDF.cpp:14-240): the particle arrays of a mesh
  * file written by lpm_mesh_write_nc or lpm_bve_checkpoint, in the layout of
- * lpm_icos_tree_init.  Call with NULL arrays to get counts {nverts, nedges,
+ * lpm_seed_tree_init.  Call with NULL arrays to get counts {nverts, nedges,
  * nfaces, nleaves} (nedges -1 when the file holds no edges); fverts and
- * fparent need a file with topology. */
+ * fparent need a file with topology; fverts receives the file's nfaceverts
+ * per face (4 when nverts = nleaves + 2, the cubed sphere; else 3). */
 int lpm_mesh_read_nc(const char* path, double* vx, double* vlag, double* fx, double* flag, double* farea,
                      uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts);
 

@@ -1,7 +1,7 @@
 // lpm_bve.hpp -- C++ host interface over the C ABI of lpm_bve.h, in the
 // shape of the reference's BVE classes:
 //
-//   MeshSeed<IcosTriSphereSeed> seed;                       src/LpmMeshSeed.hpp:181-224
+//   MeshSeed<IcosTriSphereSeed> seed;  (or CubedSphereSeed) src/LpmMeshSeed.hpp:181-224
 //   seed.setMaxAllocations(nv, ne, nf, depth);             src/LpmMeshSeed.cpp:296-305
 //   BVESphere<IcosTriSphereSeed> sphere(nv, ne, nf);        src/LpmBVESphere.hpp:40
 //   sphere.treeInit(depth, seed);                           src/LpmPolyMesh2d.hpp:171
@@ -42,10 +42,19 @@ inline void check(int rc, const char* what) {
   if (rc != LPM_OK) throw std::runtime_error(std::string(what) + ": " + lpm_last_error());
 }
 
+// The spherical seeds (src/LpmMeshSeed.hpp:111-147).
 struct IcosTriSphereSeed {
   static constexpr int nverts = 12, nfaces = 20, nedges = 30, nfaceverts = 3;
+  static constexpr int kind = LPM_SEED_ICOS_TRI;
   static std::string idString() { return "IcosTriSphereSeed"; }
   static std::string faceStr() { return "tri"; }
+};
+
+struct CubedSphereSeed {
+  static constexpr int nverts = 8, nfaces = 6, nedges = 12, nfaceverts = 4;
+  static constexpr int kind = LPM_SEED_CUBED_SPHERE;
+  static std::string idString() { return "CubedSphereSeed"; }
+  static std::string faceStr() { return "quad"; }
 };
 
 template <typename SeedType>
@@ -55,7 +64,14 @@ struct MeshSeed {
   explicit MeshSeed(const std::string& seed_file) : path(seed_file) {}
   static std::string idString() { return SeedType::idString(); }
   void setMaxAllocations(Index& nboundary, Index& nedges, Index& nfaces, const int lev) const {
-    check(lpm_icos_alloc(lev, &nboundary, &nedges, &nfaces), "setMaxAllocations");
+    check(lpm_seed_alloc(SeedType::kind, lev, &nboundary, &nedges, &nfaces), "setMaxAllocations");
+  }
+  // PolyMesh2d<SeedType>::treeInit arrays (fverts [nfaces][nfaceverts]).
+  void treeInit(int lev, double* vx, double* vlag, double* fx, double* flag, double* farea, uint8_t* fmask,
+                int64_t* counts) const {
+    check(lpm_seed_tree_init(SeedType::kind, lev, path.empty() ? nullptr : path.c_str(), vx, vlag, fx, flag, farea,
+                             fmask, nullptr, nullptr, counts),
+          "treeInit");
   }
 };
 
@@ -231,10 +247,8 @@ class BVESphere {
     face_area.assign(nf, 0);
     face_mask.assign(nf, 0);
     int64_t counts[4];
-    check(lpm_icos_tree_init(depth, seed.path.empty() ? nullptr : seed.path.c_str(), vert_x.data(),
-                             vert_lag.data(), face_x.data(), face_lag.data(), face_area.data(), face_mask.data(),
-                             nullptr, nullptr, counts),
-          "treeInit");
+    seed.treeInit(depth, vert_x.data(), vert_lag.data(), face_x.data(), face_lag.data(), face_area.data(),
+                  face_mask.data(), counts);
     nverts_ = counts[0];
     nfaces_ = counts[2];
     nleaves_ = counts[3];
@@ -326,8 +340,9 @@ class BVESphere {
     nfaces_ = sz[1];
     nleaves_ = sz[2];
     depth_ = -1;
+    const int64_t per_level = SeedType::nfaces / 2;  // nverts = 2 + (nfaces/2) 4^level on both seeds
     for (int lev = 0; lev <= 12; ++lev)
-      if (2 + 10 * (int64_t(1) << (2 * lev)) == nverts_) depth_ = lev;
+      if (2 + per_level * (int64_t(1) << (2 * lev)) == nverts_) depth_ = lev;
     vert_x.assign(3 * nverts_, 0);
     face_x.assign(3 * nfaces_, 0);
     rel_vort_verts.assign(nverts_, 0);
@@ -411,9 +426,7 @@ class SpherePoisson {
     face_area.assign(nf, 0);
     face_mask.assign(nf, 0);
     int64_t counts[4];
-    check(lpm_icos_tree_init(depth, seed.path.empty() ? nullptr : seed.path.c_str(), vert_x.data(), nullptr,
-                             face_x.data(), nullptr, face_area.data(), face_mask.data(), nullptr, nullptr, counts),
-          "treeInit");
+    seed.treeInit(depth, vert_x.data(), nullptr, face_x.data(), nullptr, face_area.data(), face_mask.data(), counts);
     nverts_ = counts[0];
     nfaces_ = counts[2];
     nleaves_ = counts[3];

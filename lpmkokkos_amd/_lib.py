@@ -18,6 +18,7 @@ HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "lpm_bve.h")
 LPM_OK = 0
 PREC_FP64, PREC_FP32C = 0, 1
 SCHEME_REF, SCHEME_LAGR = 0, 1
+SEED_ICOS_TRI, SEED_CUBED_SPHERE = 0, 1
 VTK_LEGACY_ASCII, VTK_LEGACY_BINARY, VTK_XML_ASCII, VTK_XML_BINARY = 0, 1, 2, 3
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -47,6 +48,10 @@ SIGNATURES = {
     "lpm_device_count": [C.POINTER(C.c_int)],
     "lpm_icos_alloc": [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "lpm_icos_tree_init": [C.c_int, C.c_char_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
+    "lpm_seed_alloc": [C.c_int, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+    "lpm_seed_tree_init": [C.c_int, C.c_int, C.c_char_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
+    "lpm_seed_mesh_write_nc": [C.c_int, C.c_int, C.c_char_p, C.c_char_p],
+    "lpm_seed_mesh_write_vtk": [C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int],
     "lpm_ic_gauss_params": [C.c_uint64, C.c_int, C.c_double, _dp, _dp],
     "lpm_ic_eval": [_dp, C.c_int64, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dpn, _dpn,
                     C.c_double],

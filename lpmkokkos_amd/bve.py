@@ -37,10 +37,15 @@ PI = 3.1415926535897932384626433832795027975
 class IcosTriSphereSeed:
     """src/LpmMeshSeed.hpp:132-147."""
     nverts, nfaces, nedges, nfaceverts = 12, 20, 30, 3
+    kind = _lib.SEED_ICOS_TRI
 
     @staticmethod
     def idString() -> str:
         return "IcosTriSphereSeed"
+
+    @staticmethod
+    def faceStr() -> str:
+        return "tri"
 
     @staticmethod
     def nVerticesAtTreeLevel(lev: int) -> int:
@@ -51,22 +56,50 @@ class IcosTriSphereSeed:
         return 20 * 4 ** lev
 
 
+class CubedSphereSeed:
+    """src/LpmMeshSeed.hpp:111-126: quadrilateral faces; a divided face's
+    centre becomes its children's shared corner."""
+    nverts, nfaces, nedges, nfaceverts = 8, 6, 12, 4
+    kind = _lib.SEED_CUBED_SPHERE
+
+    @staticmethod
+    def idString() -> str:
+        return "CubedSphereSeed"
+
+    @staticmethod
+    def faceStr() -> str:
+        return "quad"
+
+    @staticmethod
+    def nVerticesAtTreeLevel(lev: int) -> int:
+        return 2 + 6 * 4 ** lev
+
+    @staticmethod
+    def nFacesAtTreeLevel(lev: int) -> int:
+        return 6 * 4 ** lev
+
+
 class MeshSeed:
-    """MeshSeed<SeedType>: the built-in icosahedral seed, or a seed file in the
-    reference's format when ``path`` is given."""
+    """MeshSeed<SeedType>: the built-in seed of ``seed_type`` (IcosTriSphereSeed
+    or CubedSphereSeed), or a seed file in the reference's format when
+    ``path`` is given."""
 
     def __init__(self, seed_type=IcosTriSphereSeed, path: str | None = None):
-        if seed_type is not IcosTriSphereSeed:
-            raise NotImplementedError("only IcosTriSphereSeed is built")
+        if seed_type not in (IcosTriSphereSeed, CubedSphereSeed):
+            raise NotImplementedError("the spherical seeds are IcosTriSphereSeed and CubedSphereSeed")
         self.seed_type = seed_type
         self.path = path
+
+    @property
+    def kind(self) -> int:
+        return self.seed_type.kind
 
     def idString(self) -> str:
         return self.seed_type.idString()
 
     def setMaxAllocations(self, lev: int):
         nv, ne, nf = C.c_int64(), C.c_int64(), C.c_int64()
-        call("lpm_icos_alloc", lev, C.byref(nv), C.byref(ne), C.byref(nf))
+        call("lpm_seed_alloc", self.kind, lev, C.byref(nv), C.byref(ne), C.byref(nf))
         return nv.value, ne.value, nf.value
 
 
@@ -83,10 +116,11 @@ class TreeMesh:
         self.flag = np.zeros((nf, 3))
         self.farea = np.zeros(nf)
         self.fmask = np.zeros(nf, np.uint8)
-        self.fverts = np.zeros((nf, 3), np.int32)
+        self.seed_type = seed.seed_type
+        self.fverts = np.zeros((nf, seed.seed_type.nfaceverts), np.int32)
         self.fparent = np.zeros(nf, np.int32)
         counts = np.zeros(4, np.int64)
-        call("lpm_icos_tree_init", level, seed.path.encode() if seed.path else None,
+        call("lpm_seed_tree_init", seed.kind, level, seed.path.encode() if seed.path else None,
              *(_lib.ptr(a) for a in (self.vx, self.vlag, self.fx, self.flag, self.farea, self.fmask, self.fverts,
                                      self.fparent, counts)))
         self.nverts, self.nedges, self.nfaces, self.nleaves = (int(c) for c in counts)
@@ -100,16 +134,18 @@ class TreeMesh:
         arrays of a mesh file (write_mesh_nc or a checkpoint with topology)."""
         counts = np.zeros(4, np.int64)
         call("lpm_mesh_read_nc", path.encode(), *([None] * 8), _lib.ptr(counts))
-        nv, ne, nf = (int(c) for c in counts[:3])
+        nv, ne, nf, nl = (int(c) for c in counts)
+        seed_type = CubedSphereSeed if nv == nl + 2 else IcosTriSphereSeed
         m = cls.__new__(cls)
+        m.seed_type = seed_type
         m.vx, m.vlag = np.zeros((nv, 3)), np.zeros((nv, 3))
         m.fx, m.flag = np.zeros((nf, 3)), np.zeros((nf, 3))
         m.farea, m.fmask = np.zeros(nf), np.zeros(nf, np.uint8)
-        m.fverts, m.fparent = np.zeros((nf, 3), np.int32), np.zeros(nf, np.int32)
+        m.fverts, m.fparent = np.zeros((nf, seed_type.nfaceverts), np.int32), np.zeros(nf, np.int32)
         call("lpm_mesh_read_nc", path.encode(),
              *(_lib.ptr(a) for a in (m.vx, m.vlag, m.fx, m.flag, m.farea, m.fmask, m.fverts, m.fparent, counts)))
         m.nverts, m.nedges, m.nfaces, m.nleaves = (int(c) for c in counts)
-        m.level = next((lev for lev in range(13) if IcosTriSphereSeed.nVerticesAtTreeLevel(lev) == nv), -1)
+        m.level = next((lev for lev in range(13) if seed_type.nVerticesAtTreeLevel(lev) == nv), -1)
         return m
 
 
@@ -117,14 +153,15 @@ def write_mesh_vtk(level: int, path: str, fmt: int = _lib.VTK_LEGACY_ASCII, seed
     """PolyMesh2d::outputVtk (src/LpmPolyMesh2d.cpp:79-85) of the level-`level`
     treeInit mesh: vertices, leaf faces and their areas."""
     seed = seed or MeshSeed()
-    call("lpm_mesh_write_vtk", int(level), seed.path.encode() if seed.path else None, path.encode(), int(fmt))
+    call("lpm_seed_mesh_write_vtk", seed.kind, int(level), seed.path.encode() if seed.path else None, path.encode(),
+         int(fmt))
 
 
 def write_mesh_nc(level: int, path: str, seed: MeshSeed | None = None):
     """NcWriter::writePolymesh (src/LpmNetCDF_Impl.hpp:46-218) of the level-`level`
     treeInit mesh, as a netCDF classic (64-bit offset) file."""
     seed = seed or MeshSeed()
-    call("lpm_mesh_write_nc", int(level), seed.path.encode() if seed.path else None, path.encode())
+    call("lpm_seed_mesh_write_nc", seed.kind, int(level), seed.path.encode() if seed.path else None, path.encode())
 
 
 # ---------------------------------------------------------------------------
@@ -403,8 +440,9 @@ class _Faces:
 
 
 class BVESphere:
-    """BVESphere<IcosTriSphereSeed> (src/LpmBVESphere.hpp:18-80) with fields
-    resident on device `device`."""
+    """BVESphere<Seed> (src/LpmBVESphere.hpp:18-80; the icosahedral and the
+    cubed-sphere seeds, src/LpmBVESphere.cpp:201-202) with fields resident on
+    device `device`; the seed comes with treeInit."""
 
     def __init__(self, nmaxverts: int, nmaxedges: int, nmaxfaces: int, nq: int = 0, device: int = 0):
         self.nmax = (nmaxverts, nmaxedges, nmaxfaces)
@@ -498,9 +536,12 @@ class BVESphere:
         self.t, self.Omega = self.ctx.restore(fname)
         self._omg_set = True
         nv = self.ctx.nv
-        level = next(lev for lev in range(13) if IcosTriSphereSeed.nVerticesAtTreeLevel(lev) == nv)
-        if self.mesh is None or self.mesh.level != level:
-            self.mesh = TreeMesh(level)
+        for seed_type in (IcosTriSphereSeed, CubedSphereSeed):
+            level = next((lev for lev in range(13) if seed_type.nVerticesAtTreeLevel(lev) == nv), None)
+            if level is not None:
+                break
+        if self.mesh is None or self.mesh.level != level or self.mesh.seed_type is not seed_type:
+            self.mesh = TreeMesh(level, MeshSeed(seed_type))
 
     def avg_mesh_size_radians(self) -> float:
         return math.sqrt(4 * PI / self.nfacesHost())

@@ -3,10 +3,14 @@
 //
 //   bve_driver [-d depth] [-dt dt] [-tf tfinal] [-ic sbr|rh] [-scheme ref|lagr]
 //              [-prec fp64|fp32c] [-f output_interval] [-o case_name]
+//              [-seed cubed|icos]
+//
+// The mesh is the cubed sphere by default, as in the reference driver
+// (seed_type = CubedSphereSeed); -seed icos takes the icosahedral seed.
 //
 // With -o, the state is written as XML VTK polydata at t = 0 and every
 // output_interval steps, named as the reference's driver names its files
-// (case_name + "tri" + depth + "_dt" + dt + "_" + 4-digit step + ".vtp",
+// (case_name + "quad" or "tri" + depth + "_dt" + dt + "_" + 4-digit step + ".vtp",
 // tests/LpmBVETest.cpp:105,236-237); for solid-body rotation the velocity and
 // position errors are added as vertex_/face_velocity_error and
 // vertex_/face_position_error (:67-70, :180-217), after the tracer "u dot x".
@@ -64,30 +68,26 @@ Real rh_phase(const std::vector<Real>& x, const std::vector<Real>& zeta, const s
   return std::atan2(s, c) / n;
 }
 
-}  // namespace
-
-int main(int argc, char* argv[]) {
+struct Options {
   int depth = 3, output_interval = 0;
   Real dt = 0.01, tfinal = 1.0;
   std::string ic_name = "sbr", case_name;
   int scheme = LPM_SCHEME_REF, prec = LPM_PREC_FP64;
-  for (int i = 1; i < argc; ++i) {
-    const std::string t = argv[i];
-    if (t == "-d" && i + 1 < argc) depth = std::stoi(argv[++i]);
-    else if (t == "-dt" && i + 1 < argc) dt = std::stod(argv[++i]);
-    else if (t == "-tf" && i + 1 < argc) tfinal = std::stod(argv[++i]);
-    else if (t == "-f" && i + 1 < argc) output_interval = std::stoi(argv[++i]);
-    else if (t == "-ic" && i + 1 < argc) ic_name = argv[++i];
-    else if (t == "-o" && i + 1 < argc) case_name = argv[++i];
-    else if (t == "-scheme" && i + 1 < argc) scheme = std::string(argv[++i]) == "lagr" ? LPM_SCHEME_LAGR : LPM_SCHEME_REF;
-    else if (t == "-prec" && i + 1 < argc) prec = std::string(argv[++i]) == "fp32c" ? LPM_PREC_FP32C : LPM_PREC_FP64;
-  }
-  if (prec == LPM_PREC_FP32C) scheme = LPM_SCHEME_LAGR;
+};
+
+template <typename SeedType>
+int run(Options opt) {
+  int depth = opt.depth;
+  const int output_interval = opt.output_interval;
+  Real dt = opt.dt;
+  const Real tfinal = opt.tfinal;
+  const std::string ic_name = opt.ic_name, case_name = opt.case_name;
+  const int scheme = opt.scheme, prec = opt.prec;
   try {
-    MeshSeed<IcosTriSphereSeed> seed;
+    MeshSeed<SeedType> seed;
     Index nmaxverts, nmaxedges, nmaxfaces;
     seed.setMaxAllocations(nmaxverts, nmaxedges, nmaxfaces, depth);
-    BVESphere<IcosTriSphereSeed> sphere(nmaxverts, nmaxedges, nmaxfaces, 0);
+    BVESphere<SeedType> sphere(nmaxverts, nmaxedges, nmaxfaces, 0);
     sphere.treeInit(depth, seed);
     const Real Omega = 2 * PI;
     std::shared_ptr<VorticityInitialCondition> relvort;
@@ -145,14 +145,14 @@ int main(int argc, char* argv[]) {
       std::copy(udotx.begin(), udotx.end(), sphere.tracer_verts[dotprod].begin());
       velocity_tangent(udotx, sphere.face_x, sphere.vel_faces);
       std::copy(udotx.begin(), udotx.end(), sphere.tracer_faces[dotprod].begin());
-      std::vector<BVESphere<IcosTriSphereSeed>::VtkArray> extra;
+      std::vector<typename BVESphere<SeedType>::VtkArray> extra;
       if (!rh) {
         const Real c = std::cos(Omega * sphere.t), s = std::sin(Omega * sphere.t);
         auto errors = [&](const std::vector<Real>& x, const std::vector<Real>& u, const std::vector<Real>& lag,
                           bool faces, const char* where) {
           const size_t n = x.size() / 3;
-          BVESphere<IcosTriSphereSeed>::VtkArray ve{std::string(where) + "_velocity_error", 3, faces, std::vector<Real>(3 * n)};
-          BVESphere<IcosTriSphereSeed>::VtkArray pe{std::string(where) + "_position_error", 3, faces, std::vector<Real>(3 * n)};
+          typename BVESphere<SeedType>::VtkArray ve{std::string(where) + "_velocity_error", 3, faces, std::vector<Real>(3 * n)};
+          typename BVESphere<SeedType>::VtkArray pe{std::string(where) + "_position_error", 3, faces, std::vector<Real>(3 * n)};
           for (size_t i = 0; i < n; ++i) {
             const Real* a = &lag[3 * i];
             const Real ex[3] = {a[0] * c - a[1] * s, a[1] * c + a[0] * s, a[2]};
@@ -168,7 +168,7 @@ int main(int argc, char* argv[]) {
         errors(sphere.face_x, sphere.vel_faces, lag_faces, true, "face");
       }
       std::ostringstream ss;
-      ss << case_name << IcosTriSphereSeed::faceStr() << depth << "_dt" << dt << "_" << std::setfill('0')
+      ss << case_name << SeedType::faceStr() << depth << "_dt" << dt << "_" << std::setfill('0')
          << std::setw(4) << step << ".vtp";
       sphere.outputVtk(ss.str(), extra);
     };
@@ -236,4 +236,30 @@ int main(int argc, char* argv[]) {
     std::fprintf(stderr, "bve_driver: %s\n", e.what());
     return 2;
   }
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  Options opt;
+  std::string seed_name = "cubed";  // the reference driver's seed_type (tests/LpmBVETest.cpp:42)
+  for (int i = 1; i < argc; ++i) {
+    const std::string t = argv[i];
+    if (t == "-d" && i + 1 < argc) opt.depth = std::stoi(argv[++i]);
+    else if (t == "-dt" && i + 1 < argc) opt.dt = std::stod(argv[++i]);
+    else if (t == "-tf" && i + 1 < argc) opt.tfinal = std::stod(argv[++i]);
+    else if (t == "-f" && i + 1 < argc) opt.output_interval = std::stoi(argv[++i]);
+    else if (t == "-ic" && i + 1 < argc) opt.ic_name = argv[++i];
+    else if (t == "-o" && i + 1 < argc) opt.case_name = argv[++i];
+    else if (t == "-seed" && i + 1 < argc) seed_name = argv[++i];
+    else if (t == "-scheme" && i + 1 < argc) opt.scheme = std::string(argv[++i]) == "lagr" ? LPM_SCHEME_LAGR : LPM_SCHEME_REF;
+    else if (t == "-prec" && i + 1 < argc) opt.prec = std::string(argv[++i]) == "fp32c" ? LPM_PREC_FP32C : LPM_PREC_FP64;
+  }
+  if (opt.prec == LPM_PREC_FP32C) opt.scheme = LPM_SCHEME_LAGR;
+  if (seed_name == "icos") return run<IcosTriSphereSeed>(opt);
+  if (seed_name != "cubed") {
+    std::fprintf(stderr, "bve_driver: -seed must be icos or cubed\n");
+    return 2;
+  }
+  return run<CubedSphereSeed>(opt);
 }

@@ -2533,12 +2533,18 @@ int ctx_tree(const lpm_bve_ctx* c, int level, bool required, std::unique_ptr<Tre
     return LPM_OK;
   }
   if (level > 10) return fail(LPM_ERR_ARG, std::string(who) + ": level must be in [0, 10]");
-  int64_t nvl = IcosTriSphereSeed::nVerticesAtTreeLevel(level), nfl = 0;
-  for (int l = 0; l <= level; ++l) nfl += IcosTriSphereSeed::nFacesAtTreeLevel(l);
-  if (nvl != c->nv || nfl != c->nf)
+  // the seed whose level-`level` tree has the state's sizes (the icosahedral
+  // and cubed-sphere trees never share a size: 10 4^a != 6 4^b)
+  int kind = -1;
+  for (int k : {kSeedIcosTri, kSeedCubedSphere}) {
+    int64_t nvl = 0, nel = 0, nfl = 0;
+    seed_max_allocations(k, level, nvl, nel, nfl);
+    if (nvl == c->nv && nfl == c->nf) kind = k;
+  }
+  if (kind < 0)
     return fail(LPM_ERR_ARG, std::string(who) + ": the state's sizes are not those of tree level " + std::to_string(level));
   try {
-    out.reset(new TreeMesh(build_icos_tree(level, builtin_icos_seed())));
+    out.reset(new TreeMesh(build_tree(kind, level)));
   } catch (const std::exception& e) {
     return fail(LPM_ERR_NOMEM, std::string(who) + ": " + e.what());
   }

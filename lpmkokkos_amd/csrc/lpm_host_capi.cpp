@@ -2,7 +2,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <exception>
+#include <memory>
 #include <string>
 
 #include "lpm_bve.h"
@@ -33,93 +35,119 @@ const char* lpm_last_error(void) { return g_last_error.c_str(); }
 
 int lpm_version(void) { return 10000; }
 
-int lpm_icos_alloc(int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces) {
-  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_icos_alloc: level must be in [0, 12]");
+}  // extern "C"
+
+namespace {
+
+// The tree of seed `kind` at `level` from the built-in seed or a seed file.
+int make_tree(int kind, int level, const char* seed_path, const char* who, std::unique_ptr<TreeMesh>& out) {
+  if (kind != kSeedIcosTri && kind != kSeedCubedSphere) return fail(LPM_ERR_ARG, std::string(who) + ": unknown seed kind");
+  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, std::string(who) + ": level must be in [0, 12]");
+  try {
+    std::string err;
+    if (kind == kSeedCubedSphere) {
+      QuadSeedTables file_seed;
+      const QuadSeedTables* seed = &builtin_cubed_seed();
+      if (seed_path) {
+        if (!read_cubed_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, std::string(who) + ": " + err);
+        seed = &file_seed;
+      }
+      out.reset(new TreeMesh(build_cubed_tree(level, *seed)));
+    } else {
+      SeedTables file_seed;
+      const SeedTables* seed = &builtin_icos_seed();
+      if (seed_path) {
+        if (!read_icos_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, std::string(who) + ": " + err);
+        seed = &file_seed;
+      }
+      out.reset(new TreeMesh(build_icos_tree(level, *seed)));
+    }
+  } catch (const std::exception& e) {
+    return fail(LPM_ERR_NOMEM, std::string(who) + ": " + e.what());
+  }
+  return LPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpm_seed_alloc(int kind, int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces) {
+  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_seed_alloc: level must be in [0, 12]");
   int64_t v, e, f;
-  icos_max_allocations(level, v, e, f);
+  if (!seed_max_allocations(kind, level, v, e, f)) return fail(LPM_ERR_ARG, "lpm_seed_alloc: unknown seed kind");
   if (nverts) *nverts = v;
   if (nedges) *nedges = e;
   if (nfaces) *nfaces = f;
   return ok();
 }
 
+int lpm_icos_alloc(int level, int64_t* nverts, int64_t* nedges, int64_t* nfaces) {
+  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_icos_alloc: level must be in [0, 12]");
+  return lpm_seed_alloc(kSeedIcosTri, level, nverts, nedges, nfaces);
+}
+
+int lpm_seed_tree_init(int kind, int level, const char* seed_path, double* vx, double* vlag, double* fx, double* flag,
+                       double* farea, uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts) {
+  std::unique_ptr<TreeMesh> tm;
+  const int rc = make_tree(kind, level, seed_path, "lpm_seed_tree_init", tm);
+  if (rc) return rc;
+  const TreeMesh& m = *tm;
+  const int64_t nv = m.nverts(), nf = m.nfaces();
+  if (vx) std::memcpy(vx, m.vert_phys.data(), sizeof(double) * 3 * nv);
+  if (vlag) std::memcpy(vlag, m.vert_lag.data(), sizeof(double) * 3 * nv);
+  if (fx) std::memcpy(fx, m.face_phys.data(), sizeof(double) * 3 * nf);
+  if (flag) std::memcpy(flag, m.face_lag.data(), sizeof(double) * 3 * nf);
+  if (farea) std::memcpy(farea, m.face_area.data(), sizeof(double) * nf);
+  if (fmask) std::memcpy(fmask, m.face_mask.data(), nf);
+  if (fverts)
+    for (int64_t i = 0; i < m.nfaceverts * nf; ++i) fverts[i] = m.face_verts[i];
+  if (fparent)
+    for (int64_t i = 0; i < nf; ++i) fparent[i] = m.face_parent[i];
+  if (counts) {
+    counts[0] = nv;
+    counts[1] = m.n_edges;
+    counts[2] = nf;
+    counts[3] = m.nleaves();
+  }
+  return ok();
+}
+
 int lpm_icos_tree_init(int level, const char* seed_path, double* vx, double* vlag, double* fx, double* flag,
                        double* farea, uint8_t* fmask, int32_t* fverts, int32_t* fparent, int64_t* counts) {
-  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_icos_tree_init: level must be in [0, 12]");
-  try {
-    SeedTables file_seed;
-    const SeedTables* seed = &builtin_icos_seed();
-    if (seed_path) {
-      std::string err;
-      if (!read_icos_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, "lpm_icos_tree_init: " + err);
-      seed = &file_seed;
-    }
-    const TreeMesh m = build_icos_tree(level, *seed);
-    const int64_t nv = m.nverts(), nf = m.nfaces();
-    if (vx) std::memcpy(vx, m.vert_phys.data(), sizeof(double) * 3 * nv);
-    if (vlag) std::memcpy(vlag, m.vert_lag.data(), sizeof(double) * 3 * nv);
-    if (fx) std::memcpy(fx, m.face_phys.data(), sizeof(double) * 3 * nf);
-    if (flag) std::memcpy(flag, m.face_lag.data(), sizeof(double) * 3 * nf);
-    if (farea) std::memcpy(farea, m.face_area.data(), sizeof(double) * nf);
-    if (fmask) std::memcpy(fmask, m.face_mask.data(), nf);
-    if (fverts)
-      for (int64_t i = 0; i < 3 * nf; ++i) fverts[i] = m.face_verts[i];
-    if (fparent)
-      for (int64_t i = 0; i < nf; ++i) fparent[i] = m.face_parent[i];
-    if (counts) {
-      counts[0] = nv;
-      counts[1] = m.n_edges;
-      counts[2] = nf;
-      counts[3] = m.nleaves();
-    }
-  } catch (const std::exception& e) {
-    return fail(LPM_ERR_NOMEM, std::string("lpm_icos_tree_init: ") + e.what());
-  }
+  return lpm_seed_tree_init(kSeedIcosTri, level, seed_path, vx, vlag, fx, flag, farea, fmask, fverts, fparent, counts);
+}
+
+int lpm_seed_mesh_write_nc(int kind, int level, const char* seed_path, const char* path) {
+  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_write_nc: null path");
+  std::unique_ptr<TreeMesh> tm;
+  const int rc = make_tree(kind, level, seed_path, "lpm_mesh_write_nc", tm);
+  if (rc) return rc;
+  nc::Writer w;
+  add_mesh_vars(w, *tm, nullptr, nullptr);
+  std::string err;
+  if (!w.write(path, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_nc: " + err);
   return ok();
 }
 
 int lpm_mesh_write_nc(int level, const char* seed_path, const char* path) {
-  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_mesh_write_nc: level must be in [0, 12]");
-  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_write_nc: null path");
-  try {
-    SeedTables file_seed;
-    const SeedTables* seed = &builtin_icos_seed();
-    if (seed_path) {
-      std::string err;
-      if (!read_icos_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_nc: " + err);
-      seed = &file_seed;
-    }
-    const TreeMesh m = build_icos_tree(level, *seed);
-    nc::Writer w;
-    add_mesh_vars(w, m, nullptr, nullptr);
-    std::string err;
-    if (!w.write(path, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_nc: " + err);
-  } catch (const std::exception& e) {
-    return fail(LPM_ERR_NOMEM, std::string("lpm_mesh_write_nc: ") + e.what());
-  }
+  return lpm_seed_mesh_write_nc(kSeedIcosTri, level, seed_path, path);
+}
+
+int lpm_seed_mesh_write_vtk(int kind, int level, const char* seed_path, const char* path, int format) {
+  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: null path");
+  if (format < LPM_VTK_LEGACY_ASCII || format > LPM_VTK_XML_BINARY)
+    return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: unknown format");
+  std::unique_ptr<TreeMesh> tm;
+  const int rc = make_tree(kind, level, seed_path, "lpm_mesh_write_vtk", tm);
+  if (rc) return rc;
+  std::string err;
+  if (!write_mesh_vtk(path, *tm, format, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_vtk: " + err);
   return ok();
 }
 
 int lpm_mesh_write_vtk(int level, const char* seed_path, const char* path, int format) {
-  if (level < 0 || level > 12) return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: level must be in [0, 12]");
-  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: null path");
-  if (format < LPM_VTK_LEGACY_ASCII || format > LPM_VTK_XML_BINARY)
-    return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: unknown format");
-  try {
-    SeedTables file_seed;
-    const SeedTables* seed = &builtin_icos_seed();
-    if (seed_path) {
-      std::string err;
-      if (!read_icos_seed(seed_path, file_seed, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_vtk: " + err);
-      seed = &file_seed;
-    }
-    const TreeMesh m = build_icos_tree(level, *seed);
-    std::string err;
-    if (!write_mesh_vtk(path, m, format, err)) return fail(LPM_ERR_IO, "lpm_mesh_write_vtk: " + err);
-  } catch (const std::exception& e) {
-    return fail(LPM_ERR_NOMEM, std::string("lpm_mesh_write_vtk: ") + e.what());
-  }
-  return ok();
+  return lpm_seed_mesh_write_vtk(kSeedIcosTri, level, seed_path, path, format);
 }
 
 int lpm_mesh_read_nc(const char* path, double* vx, double* vlag, double* fx, double* flag, double* farea,
@@ -156,7 +184,8 @@ int lpm_mesh_read_nc(const char* path, double* vx, double* vlag, double* fx, dou
   };
   if (!get_d("phys_crds_verts", vx, 3 * nv) || !get_d("lag_crds_verts", vlag, 3 * nv) ||
       !get_d("phys_crds_faces", fx, 3 * nf) || !get_d("lag_crds_faces", flag, 3 * nf) ||
-      !get_d("face_area", farea, nf) || !get_i("face_verts", fverts, 3 * nf) || !get_i("face_parents", fparent, nf))
+      !get_d("face_area", farea, nf) || !get_i("face_verts", fverts, std::max<int64_t>(r.dim("nfaceverts"), 0) * nf) ||
+      !get_i("face_parents", fparent, nf))
     return fail(LPM_ERR_IO, "lpm_mesh_read_nc: missing or mis-sized variable " + err);
   if (fmask) std::memcpy(fmask, mask.data(), nf);
   if (counts) {

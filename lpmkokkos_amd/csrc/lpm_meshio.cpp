@@ -15,7 +15,7 @@ void add_mesh_vars(nc::Writer& w, const TreeMesh& m, const double* phys_v, const
   const int nverts = w.dim("nverts", m.nverts());
   const int nedges = w.dim("nedges", m.n_edges);
   const int nfaces = w.dim("nfaces", m.nfaces());
-  const int nfaceverts = w.dim("nfaceverts", IcosTriSphereSeed::nfaceverts);
+  const int nfaceverts = w.dim("nfaceverts", m.nfaceverts);
   const int two = w.dim("two", 2);
   const int four = w.dim("four", 4);
   // vertices
@@ -42,8 +42,8 @@ void add_mesh_vars(nc::Writer& w, const TreeMesh& m, const double* phys_v, const
   w.var("face_tree_level", nc::kInt, {nfaces}, m.face_level.data());
   w.var("face_parents", nc::kInt, {nfaces}, m.face_parent.data());
   w.var("face_kids", nc::kInt, {nfaces, four}, m.face_kids.data());
-  w.att_text("MeshSeed", IcosTriSphereSeed::idString());
-  w.att_text("FaceKind", "tri");
+  w.att_text("MeshSeed", m.seed_id());
+  w.att_text("FaceKind", m.face_kind());
   w.att_text("Geom", "SphereGeometry");
   w.att_int("baseTreeDepth", m.level);
 }
@@ -291,13 +291,15 @@ bool write_polydata(const std::string& path, int format, int64_t nv, const doubl
     f << "# vtk DataFile Version 3.0\n" << title << '\n' << (binary ? "BINARY" : "ASCII")
       << "\nDATASET POLYDATA\nPOINTS " << nv << " double\n";
     o.doubles(vx, 3 * nv, 3);
-    f << "POLYGONS " << nc << ' ' << 4 * nc << '\n';
-    std::vector<int32_t> conn((size_t)(4 * nc));
+    const int nfv = m.nfaceverts;
+    f << "POLYGONS " << nc << ' ' << (nfv + 1) * nc << '\n';
+    std::vector<int32_t> conn((size_t)((nfv + 1) * nc));
     for (int64_t c = 0; c < nc; ++c) {
-      conn[(size_t)(4 * c)] = 3;
-      for (int k = 0; k < 3; ++k) conn[(size_t)(4 * c + 1 + k)] = m.face_verts[(size_t)(3 * leaves[(size_t)c] + k)];
+      conn[(size_t)((nfv + 1) * c)] = nfv;
+      for (int k = 0; k < nfv; ++k)
+        conn[(size_t)((nfv + 1) * c + 1 + k)] = m.face_verts[(size_t)(nfv * leaves[(size_t)c] + k)];
     }
-    o.values(conn.data(), (int64_t)conn.size(), 4, "%d");
+    o.values(conn.data(), (int64_t)conn.size(), nfv + 1, "%d");
     auto fields = [&](const std::vector<Array>& as, int64_t n) {
       f << "FIELD FieldData " << as.size() << '\n';
       for (const Array& a : as) {
@@ -335,10 +337,11 @@ bool write_polydata(const std::string& path, int format, int64_t nv, const doubl
       << "\">\n";
     o.doubles(vx, 3 * nv, 3);
     f << "</DataArray>\n</Points>\n<Polys>\n";
-    std::vector<long long> conn((size_t)(3 * nc)), offs((size_t)nc);
+    const int nfv = m.nfaceverts;
+    std::vector<long long> conn((size_t)(nfv * nc)), offs((size_t)nc);
     for (int64_t c = 0; c < nc; ++c) {
-      for (int k = 0; k < 3; ++k) conn[(size_t)(3 * c + k)] = m.face_verts[(size_t)(3 * leaves[(size_t)c] + k)];
-      offs[(size_t)c] = 3 * (c + 1);
+      for (int k = 0; k < nfv; ++k) conn[(size_t)(nfv * c + k)] = m.face_verts[(size_t)(nfv * leaves[(size_t)c] + k)];
+      offs[(size_t)c] = nfv * (c + 1);
     }
     f << "<DataArray type=\"Int64\" Name=\"connectivity\" format=\"" << fmt << "\">\n";
     o.values(conn.data(), (int64_t)conn.size(), 9, "%lld");

@@ -1,7 +1,8 @@
 // The reference's SpherePoisson test program (tests/LpmSpherePoisson.cpp:9-55)
-// on this library, icosahedral seed: for each tree depth, build the mesh,
-// set the Y_5^4 right-hand side, solve on the GPU and print the mesh size and
-// L-infinity errors in the reference's format.
+// on this library: for each tree depth, on the icosahedral and then the
+// cubed-sphere seed, build the mesh, set the Y_5^4 right-hand side, solve on
+// the GPU and print the mesh size and L-infinity errors in the reference's
+// format.
 //
 //     poisson_driver [max_depth]     (depths 0 .. max_depth, default 3)
 #include <cstdlib>
@@ -16,15 +17,28 @@ int main(int argc, char* argv[]) {
   try {
     for (int tree_depth = 0; tree_depth <= max_depth; ++tree_depth) {
       Index nmaxverts, nmaxedges, nmaxfaces;
-      MeshSeed<IcosTriSphereSeed> triseed;
-      triseed.setMaxAllocations(nmaxverts, nmaxedges, nmaxfaces, tree_depth);
-      SpherePoisson<IcosTriSphereSeed> ic(nmaxverts, nmaxedges, nmaxfaces);
-      ic.treeInit(tree_depth, triseed);
-      ic.updateDevice();
-      ic.init();
-      std::cout << "icostri: ";
-      ic.solve();
-      ic.updateHost();
+      {
+        MeshSeed<IcosTriSphereSeed> triseed;
+        triseed.setMaxAllocations(nmaxverts, nmaxedges, nmaxfaces, tree_depth);
+        SpherePoisson<IcosTriSphereSeed> ic(nmaxverts, nmaxedges, nmaxfaces);
+        ic.treeInit(tree_depth, triseed);
+        ic.updateDevice();
+        ic.init();
+        std::cout << "icostri: ";
+        ic.solve();
+        ic.updateHost();
+      }
+      {
+        MeshSeed<CubedSphereSeed> quadseed;
+        quadseed.setMaxAllocations(nmaxverts, nmaxedges, nmaxfaces, tree_depth);
+        SpherePoisson<CubedSphereSeed> cs(nmaxverts, nmaxedges, nmaxfaces);
+        cs.treeInit(tree_depth, quadseed);
+        cs.updateDevice();
+        cs.init();
+        std::cout << "cubedsphere: ";
+        cs.solve();
+        cs.updateHost();
+      }
     }
   } catch (const std::exception& e) {
     std::cerr << "poisson_driver: " << e.what() << std::endl;

@@ -437,6 +437,236 @@ ORC_EXPORT int orc_icos_tree_init(int lev, const double* scrds, const int* sedge
 }
 
 /* ------------------------------------------------------------------ */
+/* cubed-sphere seed and quadrilateral tree:                           */
+/* MeshSeed<CubedSphereSeed> (src/LpmMeshSeed.hpp:111-126,             */
+/* src/LpmMeshSeed.cpp:20-232, 296-320, 369-379) and                   */
+/* FaceDivider<SphereGeometry,QuadFace>::divide (src/LpmFaces.cpp:251-381)*/
+/* ------------------------------------------------------------------ */
+#define CUBE_NV 8
+#define CUBE_NF 6
+#define CUBE_NE 12
+
+/* The seed format with quadrilateral faces: a header line, 14 coordinate
+ * lines (8 vertices, then 6 face centres), then the "edgeO" (12 lines),
+ * "faceverts" (6 x 4) and "faceedges" (6 x 4) sections.  0 on success. */
+ORC_EXPORT int orc_cubed_seed_read(const char* path, double* crds /*14x3*/, int* edges /*12x4*/,
+                                   int* faceverts /*6x4*/, int* faceedges /*6x4*/) {
+  FILE* f = fopen(path, "r");
+  if (!f) return -1;
+  char line[1024];
+  int lineno = 0, edgeHdr = -1, fvHdr = -1, feHdr = -1;
+  int nc = 0, ne = 0, nfv = 0, nfe = 0;
+  const int ncrds = CUBE_NV + CUBE_NF;
+  while (fgets(line, sizeof line, f)) {
+    ++lineno;
+    if (strstr(line, "edgeO")) edgeHdr = lineno;
+    if (strstr(line, "faceverts")) fvHdr = lineno;
+    if (strstr(line, "faceedges")) feHdr = lineno;
+    if (lineno > 1 && lineno < ncrds + 2) {
+      if (sscanf(line, "%lf %lf %lf", &crds[3 * nc], &crds[3 * nc + 1], &crds[3 * nc + 2]) != 3) { fclose(f); return -2; }
+      ++nc;
+    } else if (edgeHdr > 0 && lineno > edgeHdr && lineno < edgeHdr + CUBE_NE + 1) {
+      if (sscanf(line, "%d %d %d %d", &edges[4 * ne], &edges[4 * ne + 1], &edges[4 * ne + 2], &edges[4 * ne + 3]) != 4) {
+        fclose(f);
+        return -3;
+      }
+      ++ne;
+    } else if (fvHdr > 0 && lineno > fvHdr && lineno < fvHdr + CUBE_NF + 1) {
+      int* v = &faceverts[4 * nfv];
+      if (sscanf(line, "%d %d %d %d", &v[0], &v[1], &v[2], &v[3]) != 4) { fclose(f); return -4; }
+      ++nfv;
+    } else if (feHdr > 0 && lineno > feHdr && lineno < feHdr + CUBE_NF + 1) {
+      int* e = &faceedges[4 * nfe];
+      if (sscanf(line, "%d %d %d %d", &e[0], &e[1], &e[2], &e[3]) != 4) { fclose(f); return -5; }
+      ++nfe;
+    }
+  }
+  fclose(f);
+  return (nc == ncrds && ne == CUBE_NE && nfv == CUBE_NF && nfe == CUBE_NF) ? 0 : -6;
+}
+
+/* src/LpmMeshSeed.cpp:296-305 with the counts of :369-379:
+ * nverts(l) = 2 + 6 4^l, nfaces(l) = 6 4^l, nedges(l) = nverts + nfaces - 2. */
+ORC_EXPORT void orc_cubed_alloc(int lev, int* nv, int* ne, int* nf) {
+  long p = 1;
+  for (int i = 0; i < lev; ++i) p *= 4;
+  *nv = (int)(2 + 6 * p);
+  int faces = 0, edges = 0;
+  long q = 1;
+  for (int i = 0; i <= lev; ++i) {
+    const int nvi = (int)(2 + 6 * q), nfi = (int)(6 * q);
+    faces += nfi;
+    edges += nfi + nvi - 2;
+    q *= 4;
+  }
+  *nf = faces;
+  *ne = edges;
+}
+
+typedef struct {
+  orc_mesh e;     /* vertices, edges, face coordinates, areas, masks, parents, kids */
+  int (*fv4)[4];  /* face vertices */
+  int (*fe4)[4];  /* face edges */
+} orc_qmesh;
+
+static void face_insert4(orc_qmesh* q, int ctr_ind, const int* verts, const int* edges, int prt, double area) {
+  orc_mesh* m = &q->e; /* src/LpmFaces.cpp:9-27 */
+  const int ins = m->nf;
+  for (int j = 0; j < 4; ++j) {
+    q->fv4[ins][j] = verts[j];
+    q->fe4[ins][j] = edges[j];
+    m->fk[ins][j] = NULL_IND;
+  }
+  m->fctr[ins] = ctr_ind;
+  m->fpar[ins] = prt;
+  m->farea[ins] = area;
+  m->fmask[ins] = 0;
+  m->nf += 1;
+}
+
+static void quad_face_divide(orc_qmesh* q, int faceInd) { /* src/LpmFaces.cpp:251-381 */
+  orc_mesh* m = &q->e;
+  int nfv[4][4], nfe[4][4], kids[4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) nfv[i][j] = nfe[i][j] = NULL_IND;
+  const int face_insert_pt = m->nf;
+  for (int i = 0; i < 4; ++i) kids[i] = face_insert_pt + i;
+  for (int i = 0; i < 4; ++i) nfv[i][i] = q->fv4[faceInd][i];
+  for (int i = 0; i < 4; ++i) {
+    const int parentEdge = q->fe4[faceInd][i];
+    int ek0, ek1;
+    if (m->ek[parentEdge][0] > 0) {
+      ek0 = m->ek[parentEdge][0];
+      ek1 = m->ek[parentEdge][1];
+    } else {
+      ek0 = m->ne;
+      ek1 = m->ne + 1;
+      edge_divide(m, parentEdge);
+    }
+    if (faceInd == m->el[parentEdge]) {
+      nfe[i][i] = ek0;
+      m->el[ek0] = kids[i];
+      nfe[(i + 1) % 4][i] = ek1;
+      m->el[ek1] = kids[(i + 1) % 4];
+    } else {
+      nfe[i][i] = ek1;
+      m->er[ek1] = kids[i];
+      nfe[(i + 1) % 4][i] = ek0;
+      m->er[ek0] = kids[(i + 1) % 4];
+    }
+    const int c1dest = m->ed[ek0];
+    nfv[i][(i + 1) % 4] = c1dest;
+    nfv[(i + 1) % 4][i] = c1dest;
+  }
+  /* the parent's centre becomes the children's shared vertex */
+  const int pc = m->fctr[faceInd];
+  const int bndry_insert_pt = m->nv;
+  memcpy(m->vx[m->nv], m->fx[pc], sizeof(double) * 3);
+  memcpy(m->vlag[m->nv], m->flag[pc], sizeof(double) * 3);
+  m->nv += 1;
+  for (int i = 0; i < 4; ++i) nfv[i][(i + 2) % 4] = bndry_insert_pt;
+  const int edge_ins_pt = m->ne;
+  edge_insert(m, nfv[0][1], nfv[0][2], kids[0], kids[1], NULL_IND);
+  nfe[0][1] = edge_ins_pt;
+  nfe[1][3] = edge_ins_pt;
+  edge_insert(m, nfv[2][0], nfv[2][3], kids[3], kids[2], NULL_IND);
+  nfe[2][3] = edge_ins_pt + 1;
+  nfe[3][1] = edge_ins_pt + 1;
+  edge_insert(m, nfv[2][1], nfv[2][0], kids[1], kids[2], NULL_IND);
+  nfe[1][2] = edge_ins_pt + 2;
+  nfe[2][0] = edge_ins_pt + 2;
+  edge_insert(m, nfv[3][1], nfv[3][0], kids[0], kids[3], NULL_IND);
+  nfe[0][2] = edge_ins_pt + 3;
+  nfe[3][0] = edge_ins_pt + 3;
+  double ctr[4][3], lagctr[4][3], ar[4];
+  for (int i = 0; i < 4; ++i) {
+    double vc[4][3], vl[4][3];
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 3; ++k) {
+        vc[j][k] = m->vx[nfv[i][j]][k];
+        vl[j][k] = m->vlag[nfv[i][j]][k];
+      }
+    g_barycenter(ctr[i], (const double(*)[3])vc, 4);
+    g_barycenter(lagctr[i], (const double(*)[3])vl, 4);
+    ar[i] = g_polygon_area(ctr[i], (const double(*)[3])vc, 4);
+  }
+  const int crd_ins_pt = m->nfc;
+  for (int i = 0; i < 4; ++i) {
+    memcpy(m->fx[m->nfc], ctr[i], sizeof ctr[i]);
+    memcpy(m->flag[m->nfc], lagctr[i], sizeof lagctr[i]);
+    m->nfc += 1;
+    face_insert4(q, crd_ins_pt + i, nfv[i], nfe[i], faceInd, ar[i]);
+  }
+  for (int i = 0; i < 4; ++i) m->fk[faceInd][i] = kids[i];
+  m->farea[faceInd] = 0.0;
+  m->fmask[faceInd] = 1;
+}
+
+/* PolyMesh2d<CubedSphereSeed>::treeInit (src/LpmPolyMesh2d.cpp:30-44): the
+ * outputs of orc_icos_tree_init with fverts [nf][4], sized by orc_cubed_alloc. */
+ORC_EXPORT int orc_cubed_tree_init(int lev, const double* scrds, const int* sedges, const int* sfaceverts,
+                                   const int* sfaceedges, double* vx, double* vlag, double* fx, double* flag,
+                                   double* farea, unsigned char* fmask, int* fverts, int* fparent, int* counts) {
+  orc_qmesh q;
+  memset(&q, 0, sizeof q);
+  orc_mesh* m = &q.e;
+  orc_cubed_alloc(lev, &m->nmaxv, &m->nmaxe, &m->nmaxf);
+  m->vx = (double(*)[3])vx;
+  m->vlag = (double(*)[3])vlag;
+  m->fx = (double(*)[3])fx;
+  m->flag = (double(*)[3])flag;
+  m->farea = farea;
+  m->fmask = fmask;
+  m->fpar = fparent;
+  q.fv4 = (int(*)[4])fverts;
+  m->eo = malloc(sizeof(int) * m->nmaxe);
+  m->ed = malloc(sizeof(int) * m->nmaxe);
+  m->el = malloc(sizeof(int) * m->nmaxe);
+  m->er = malloc(sizeof(int) * m->nmaxe);
+  m->ep = malloc(sizeof(int) * m->nmaxe);
+  m->ek = malloc(sizeof(int) * 2 * m->nmaxe);
+  q.fe4 = malloc(sizeof(int) * 4 * m->nmaxf);
+  m->fctr = malloc(sizeof(int) * m->nmaxf);
+  m->fk = malloc(sizeof(int) * 4 * m->nmaxf);
+  for (int i = 0; i < CUBE_NV; ++i)
+    for (int j = 0; j < 3; ++j) m->vx[i][j] = m->vlag[i][j] = scrds[3 * i + j];
+  m->nv = CUBE_NV;
+  for (int i = 0; i < CUBE_NE; ++i) {
+    m->eo[i] = sedges[4 * i + 0];
+    m->ed[i] = sedges[4 * i + 1];
+    m->el[i] = sedges[4 * i + 2];
+    m->er[i] = sedges[4 * i + 3];
+    m->ep[i] = NULL_IND;
+    m->ek[i][0] = m->ek[i][1] = NULL_IND;
+  }
+  m->ne = CUBE_NE;
+  for (int i = 0; i < CUBE_NF; ++i) { /* src/LpmFaces.cpp:43-63 */
+    double vc[4][3];
+    for (int k = 0; k < 4; ++k)
+      for (int j = 0; j < 3; ++j) vc[k][j] = scrds[3 * sfaceverts[4 * i + k] + j];
+    /* MeshSeed::faceArea, src/LpmMeshSeed.cpp:307-320 */
+    face_insert4(&q, i, &sfaceverts[4 * i], &sfaceedges[4 * i], NULL_IND,
+                 g_polygon_area(&scrds[3 * (CUBE_NV + i)], (const double(*)[3])vc, 4));
+    for (int j = 0; j < 3; ++j) m->fx[i][j] = m->flag[i][j] = scrds[3 * (CUBE_NV + i) + j];
+  }
+  m->nfc = CUBE_NF;
+  for (int l = 0; l < lev; ++l) {
+    const int stop = m->nf;
+    for (int j = 0; j < stop; ++j)
+      if (!(m->fk[j][0] > 0)) quad_face_divide(&q, j);
+  }
+  int nleaves = 0;
+  for (int i = 0; i < m->nf; ++i) nleaves += !m->fmask[i];
+  counts[0] = m->nv;
+  counts[1] = m->ne;
+  counts[2] = m->nf;
+  counts[3] = nleaves;
+  free(m->eo); free(m->ed); free(m->el); free(m->er); free(m->ep); free(m->ek);
+  free(q.fe4); free(m->fctr); free(m->fk);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* pair sums: src/LpmBVEKernels.hpp:21-305                             */
 /* ------------------------------------------------------------------ */
 

@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblpm_oracle.so")
 SEED_FIXTURE = os.path.join(os.path.dirname(HERE), "tests", "golden", "icosTriSphereSeed.dat")
+CUBED_SEED_FIXTURE = os.path.join(os.path.dirname(HERE), "tests", "golden", "cubedSphereSeed.dat")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -37,6 +38,9 @@ def lib():
         L.orc_seed_read.argtypes = [C.c_char_p, _dp, _ip, _ip, _ip]
         L.orc_icos_alloc.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_icos_tree_init.argtypes = [C.c_int, _dp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _bp, _ip, _ip, _ip]
+        L.orc_cubed_seed_read.argtypes = [C.c_char_p, _dp, _ip, _ip, _ip]
+        L.orc_cubed_alloc.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_cubed_tree_init.argtypes = [C.c_int, _dp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _bp, _ip, _ip, _ip]
         L.orc_velocity.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int, C.c_int]
         L.orc_streamfn.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int]
         L.orc_velocity_chord.argtypes = [_dp, _dp, C.c_int, _dp, _dp, _dp, _bp, C.c_int, C.c_int, C.c_int]
@@ -91,24 +95,49 @@ def icos_alloc(level: int):
     return nv.value, ne.value, nf.value
 
 
-class Mesh:
-    """PolyMesh2d<IcosTriSphereSeed>::treeInit (src/LpmPolyMesh2d.cpp:30-44) output."""
+def read_cubed_seed(path: str = CUBED_SEED_FIXTURE):
+    """MeshSeed<CubedSphereSeed>::readfile (src/LpmMeshSeed.cpp:20-232)."""
+    crds = np.zeros((14, 3))
+    edges = np.zeros((12, 4), np.int32)
+    fv = np.zeros((6, 4), np.int32)
+    fe = np.zeros((6, 4), np.int32)
+    rc = lib().orc_cubed_seed_read(path.encode(), crds, edges, fv, fe)
+    if rc != 0:
+        raise RuntimeError(f"seed read failed ({rc}) for {path}")
+    return crds, edges, fv, fe
 
-    def __init__(self, level: int, seed=None):
-        crds, edges, fv, fe = seed if seed is not None else read_seed()
-        nv, ne, nf = icos_alloc(level)
+
+def cubed_alloc(level: int):
+    """MeshSeed<CubedSphereSeed>::setMaxAllocations (src/LpmMeshSeed.cpp:296-305, :369-379)."""
+    nv, ne, nf = C.c_int(), C.c_int(), C.c_int()
+    lib().orc_cubed_alloc(level, C.byref(nv), C.byref(ne), C.byref(nf))
+    return nv.value, ne.value, nf.value
+
+
+class Mesh:
+    """PolyMesh2d<Seed>::treeInit (src/LpmPolyMesh2d.cpp:30-44) output, for the
+    icosahedral triangle seed (kind "icos") or the cubed-sphere quadrilateral
+    seed (kind "cubed")."""
+
+    def __init__(self, level: int, seed=None, kind: str = "icos"):
+        cubed = kind == "cubed"
+        crds, edges, fv, fe = seed if seed is not None else (read_cubed_seed() if cubed else read_seed())
+        nv, ne, nf = cubed_alloc(level) if cubed else icos_alloc(level)
         self.level = level
+        self.kind = kind
+        self.nfaceverts = 4 if cubed else 3
         self.vx = np.zeros((nv, 3))
         self.vlag = np.zeros((nv, 3))
         self.fx = np.zeros((nf, 3))
         self.flag = np.zeros((nf, 3))
         self.farea = np.zeros(nf)
         self.fmask = np.zeros(nf, np.uint8)
-        self.fverts = np.zeros((nf, 3), np.int32)
+        self.fverts = np.zeros((nf, self.nfaceverts), np.int32)
         self.fparent = np.zeros(nf, np.int32)
         counts = np.zeros(4, np.int32)
-        lib().orc_icos_tree_init(level, crds, edges, fv, fe, self.vx, self.vlag, self.fx, self.flag, self.farea,
-                                 self.fmask, self.fverts, self.fparent, counts)
+        init = lib().orc_cubed_tree_init if cubed else lib().orc_icos_tree_init
+        init(level, crds, edges, fv, fe, self.vx, self.vlag, self.fx, self.flag, self.farea, self.fmask,
+             self.fverts, self.fparent, counts)
         self.nv, self.ne, self.nf, self.nleaves = (int(c) for c in counts)
         assert (self.nv, self.nf) == (nv, nf)
 

@@ -226,7 +226,9 @@ def read_legacy(path):
     pts = c.values(3 * n, binary, np.float64).reshape(n, 3)
     tag, nc, size = c.line().split()
     assert tag == "POLYGONS"
-    conn = c.values(int(size), binary, np.int32).reshape(int(nc), 4)
+    nsize = int(size) // int(nc)  # every polygon has the same vertex count here (3 or 4, plus the count)
+    conn = c.values(int(size), binary, np.int32).reshape(int(nc), nsize)
+    assert np.all(conn[:, 0] == nsize - 1)
     out = {"points": pts, "polys": conn, "point": {}, "cell": {}}
     while c.p < len(c.b):
         kind, count = c.line().split()
@@ -259,8 +261,9 @@ def read_xml(path):
         for el in piece.find(tag).findall("DataArray"):
             out[key][el.get("Name")] = arr(el)
     polys = {el.get("Name"): arr(el).ravel() for el in piece.find("Polys").findall("DataArray")}
-    assert np.array_equal(polys["offsets"], 3 * np.arange(1, len(polys["offsets"]) + 1))
-    out["polys"] = polys["connectivity"].reshape(-1, 3)
+    nfv = int(polys["offsets"][0])  # every polygon has the same vertex count here (3 or 4)
+    assert np.array_equal(polys["offsets"], nfv * np.arange(1, len(polys["offsets"]) + 1))
+    out["polys"] = polys["connectivity"].reshape(-1, nfv)
     assert int(piece.get("NumberOfPoints")) == len(out["points"]) and int(piece.get("NumberOfPolys")) == len(out["polys"])
     return out
 
@@ -359,8 +362,8 @@ def test_cpp_driver_writes_the_reference_files(tmp_path):
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lpmkokkos_amd", "bve_driver")
     case = str(tmp_path / "bve_test")
-    r = subprocess.run([exe, "-d", "3", "-dt", "0.01", "-tf", "0.04", "-f", "2", "-o", case, "-scheme", "lagr"],
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "-d", "3", "-dt", "0.01", "-tf", "0.04", "-f", "2", "-o", case, "-scheme", "lagr",
+                        "-seed", "icos"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     names = sorted(p.name for p in tmp_path.iterdir())
     assert names == ["bve_testtri3_dt0.01_0000.vtp", "bve_testtri3_dt0.01_0002.vtp", "bve_testtri3_dt0.01_0004.vtp"]
@@ -373,6 +376,17 @@ def test_cpp_driver_writes_the_reference_files(tmp_path):
     assert np.all(d0["point"]["vertex_position_error"] == 0) and np.abs(d0["cell"]["face_velocity_error"]).max() < 0.1
     d2 = read_xml(str(tmp_path / names[2]))
     assert 0 < np.abs(d2["cell"]["face_position_error"]).max() < 0.01
+    # the driver's default seed, the cubed sphere: "quad" files of quadrilaterals
+    qdir = tmp_path / "quad"
+    qdir.mkdir()
+    r = subprocess.run([exe, "-d", "3", "-dt", "0.01", "-tf", "0.02", "-f", "2", "-o", str(qdir / "bve_test")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    qnames = sorted(p.name for p in qdir.iterdir())
+    assert qnames == ["bve_testquad3_dt0.01_0000.vtp", "bve_testquad3_dt0.01_0002.vtp"]
+    q = read_xml(str(qdir / qnames[0]))
+    m = L.TreeMesh(3, L.MeshSeed(L.CubedSphereSeed))
+    assert np.array_equal(q["polys"], m.fverts[m.fmask == 0]) and len(q["points"]) == m.nverts
 
 
 def test_sphere_tracers_in_vtk(tmp_path):

@@ -43,9 +43,17 @@ def test_rossby_haurwitz_pattern_speed(omesh, prec):
     c.close()
 
 
-@pytest.mark.parametrize("args", [["-d", "3"], ["-d", "5", "-ic", "rh", "-scheme", "lagr", "-prec", "fp32c"],
-                                  ["-d", "4", "-scheme", "lagr", "-f", "50"]])
+@pytest.mark.parametrize("args", [["-d", "3", "-seed", "icos"],
+                                  ["-d", "5", "-seed", "icos", "-ic", "rh", "-scheme", "lagr", "-prec", "fp32c"],
+                                  ["-d", "4", "-seed", "icos", "-scheme", "lagr", "-f", "50"],
+                                  ["-d", "4"],
+                                  ["-d", "6", "-ic", "rh", "-scheme", "lagr", "-prec", "fp32c"],
+                                  ["-d", "5", "-scheme", "lagr", "-f", "50"]])
 def test_cpp_driver(args):
+    """The reference's BVE driver on this library: solid-body rotation errors
+    and the Rossby-Haurwitz pattern speed within the driver's bounds, on the
+    cubed sphere (the driver's default seed, tests/LpmBVETest.cpp:42) and on
+    the icosahedral mesh."""
     exe = os.path.join(REPO, "lpmkokkos_amd", "bve_driver")
     r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=120)
     print(r.stdout)

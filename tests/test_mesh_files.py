@@ -212,3 +212,49 @@ def test_mesh_vtk(tmp_path, fmt):
     assert np.array_equal(d["cell"]["area"].ravel(), m.farea[leaf])
     with pytest.raises(L.LpmError):
         L.write_mesh_vtk(3, path, 7)
+
+
+def test_cubed_sphere_files(tmp_path):
+    """The cubed-sphere tree (MeshSeed<CubedSphereSeed>, src/LpmFaces.cpp:251-381)
+    in the writePolymesh layout: four vertices per face, MeshSeed
+    "CubedSphereSeed", FaceKind "quad"; the arrays equal the oracle's
+    restatement bit for bit, level 0 equals the seed file's tables, and the
+    file reads back; the VTK mesh holds quadrilaterals."""
+    from oracle import oracle as O
+    from test_gpu_files import read_legacy, read_xml
+    seed = L.MeshSeed(L.CubedSphereSeed)
+    crds, sedges, sfv, sfe = O.read_cubed_seed()
+    for lev in (0, 2, 4):
+        m = L.TreeMesh(lev, seed)
+        o = O.Mesh(lev, kind="cubed")
+        for k in ("vx", "vlag", "fx", "flag", "farea", "fmask", "fverts", "fparent"):
+            assert np.array_equal(getattr(m, k), getattr(o, k)), (lev, k)
+        p = str(tmp_path / f"cubedsphere_{lev}.nc")
+        L.write_mesh_nc(lev, p, seed)
+        f, v = read(p)
+        assert dict(f.dimensions) == {"ndim": 3, "nverts": m.nverts, "nedges": m.nedges, "nfaces": m.nfaces,
+                                      "nfaceverts": 4, "two": 2, "four": 4}
+        assert f._attributes["MeshSeed"] == b"CubedSphereSeed" and f._attributes["FaceKind"] == b"quad"
+        assert np.array_equal(v["face_verts"], m.fverts) and np.array_equal(v["face_area"], m.farea)
+        if lev == 0:
+            assert np.array_equal(v["edge_origs"], sedges[:, 0]) and np.array_equal(v["edge_rights"], sedges[:, 3])
+            assert np.array_equal(v["face_verts"], sfv) and np.array_equal(v["face_edges"], sfe)
+            assert np.array_equal(m.vx, crds[:8]) and np.array_equal(m.fx, crds[8:])
+        r = L.TreeMesh.from_nc(p)
+        assert r.seed_type is L.CubedSphereSeed and r.level == lev
+        for k in ("vx", "fx", "farea", "fmask", "fverts", "fparent"):
+            assert np.array_equal(getattr(r, k), getattr(m, k)), k
+        # each face's edge k joins its vertices k and k+1 as left/right says
+        o_, d_, lf = v["edge_origs"], v["edge_dests"], v["edge_lefts"]
+        for fi in range(m.nfaces):
+            for k in range(4):
+                e, a, b = v["face_edges"][fi, k], m.fverts[fi, k], m.fverts[fi, (k + 1) % 4]
+                assert (o_[e], d_[e]) == ((a, b) if lf[e] == fi else (b, a))
+    leaf = m.fmask == 0
+    for fmt in (0, 3):
+        path = str(tmp_path / ("c.vtp" if fmt >= 2 else "c.vtk"))
+        L.write_mesh_vtk(4, path, fmt, seed)
+        d = read_xml(path) if fmt >= 2 else read_legacy(path)
+        polys = d["polys"] if fmt >= 2 else d["polys"][:, 1:]
+        assert polys.shape[1] == 4 and np.array_equal(polys, m.fverts[leaf])
+        assert np.array_equal(d["cell"]["area"].ravel(), m.farea[leaf])

@@ -19,6 +19,18 @@ POISSON_ICOS = [
 ]
 
 
+# scripts/poissonOutput.py:7-17 (serial) and :20-30 (GPU), identical: the
+# cubed-sphere rows (mesh size in degrees, psi L-inf at vertices, at faces)
+POISSON_CUBED = [
+    (82.9186, 0.2566, 0.0),
+    (41.4593, 0.898719, 0.466355),
+    (20.7296, 0.105744, 0.330876),
+    (10.3648, 0.0242884, 0.153158),
+    (5.18241, 0.00611779, 0.0500934),
+    (2.59121, 0.00152096, 0.0148438),
+]
+
+
 def sig6(a, b):
     """Equal when printed with 6 significant digits (std::cout default)."""
     return float(f"{a:.6g}") == float(f"{b:.6g}")
@@ -378,3 +390,81 @@ def test_err_norms_closed_forms(oracle_mod):
     assert abs(l1 - 0.5 * w[17] / w.sum()) < 1e-15
     assert abs(l2 - math.sqrt(0.25 * w[17] / w.sum())) < 1e-15
     assert abs(li - 0.5) < 1e-15
+
+
+@pytest.mark.parametrize("level", range(6))
+def test_poisson_table_cubed_sphere(oracle_mod, level):
+    """The cubed-sphere rows of the same published table: pins the
+    cubed-sphere seed, the quadrilateral divider (src/LpmFaces.cpp:251-381:
+    edge midpoints, the parent centre as the shared corner, barycentres and
+    four-triangle areas) and the psi sum on quadrilateral meshes."""
+    O = oracle_mod
+    m = O.Mesh(level, kind="cubed")
+    fv, _ = O.sphharm54(m.vx)
+    ff, _ = O.sphharm54(m.fx)
+    pv = O.streamfn(m.vx, m.fx, ff, m.farea, m.fmask, False)
+    pf = O.streamfn(m.fx, m.fx, ff, m.farea, m.fmask, True)
+    ev = np.abs(pv - fv / 30).max()
+    ef = np.where(m.fmask == 1, 0.0, np.abs(pf - ff / 30)).max()
+    h = math.degrees(math.sqrt(4 * PI / m.nleaves))
+    want = POISSON_CUBED[level]
+    assert sig6(h, want[0])
+    assert sig6(ev, want[1]), (ev, want[1])
+    if want[2] == 0.0:
+        assert ef < 1e-15  # psi and f vanish at the six face centres (poles and equator)
+    else:
+        assert sig6(ef, want[2]), (ef, want[2])
+
+
+@pytest.mark.parametrize("level", range(5))
+def test_cubed_alloc_and_topology(oracle_mod, level):
+    """Closed-form counts (src/LpmMeshSeed.cpp:369-379), area = 4 pi,
+    quadrilateral leaves: Euler characteristic 2, every leaf edge in two
+    leaves, children four consecutive faces of their parent, unit positions."""
+    O = oracle_mod
+    m = O.Mesh(level, kind="cubed")
+    nv, ne, nf = O.cubed_alloc(level)
+    assert m.nv == 2 + 6 * 4 ** level == nv
+    assert m.nf == sum(6 * 4 ** i for i in range(level + 1)) == nf
+    assert m.ne == ne and m.nleaves == 6 * 4 ** level
+    assert m.nv - 2 * m.nleaves + m.nleaves == 2  # V - E + F with E = 4F/2
+    assert abs(m.farea.sum() - 4 * PI) < 1e-12
+    assert np.all(m.farea[m.fmask == 1] == 0.0) and np.all(m.farea[m.fmask == 0] > 0)
+    assert np.allclose(np.linalg.norm(m.vx, axis=1), 1.0, atol=1e-15)
+    assert np.allclose(np.linalg.norm(m.fx, axis=1), 1.0, atol=1e-15)
+    lv = m.fmask == 0
+    edges = {}
+    for f in np.nonzero(lv)[0]:
+        v = m.fverts[f]
+        assert len(set(v.tolist())) == 4
+        for k in range(4):
+            key = tuple(sorted((int(v[k]), int(v[(k + 1) % 4]))))
+            edges[key] = edges.get(key, 0) + 1
+        # counter-clockwise seen from outside: (v1 - v0) x (v2 - v0) points outward
+        a, b, c = m.vx[v[0]], m.vx[v[1]], m.vx[v[2]]
+        assert np.dot(np.cross(b - a, c - a), m.fx[f]) > 0
+    assert set(edges.values()) == {2} and len(edges) == 2 * m.nleaves
+    if level:
+        kids = np.nonzero(m.fparent >= 0)[0]
+        assert np.all(m.fparent[kids].reshape(-1, 4) == m.fparent[kids][::4, None])
+        # the parent's centre is the corner its four children share
+        for p in np.unique(m.fparent[kids])[:50]:
+            ch = np.nonzero(m.fparent == p)[0]
+            common = set(m.fverts[ch[0]].tolist())
+            for c in ch[1:]:
+                common &= set(m.fverts[c].tolist())
+            assert len(common) == 1
+            assert np.array_equal(m.vx[common.pop()], m.fx[p])
+
+
+def test_cubed_seed_values(oracle_mod):
+    """Seed file: 8 cube corners and 6 face centres on the unit sphere; each
+    face centre is the normalised mean of its corners; every edge bounds the
+    two faces it names (mesh_seeds/cubedSphereSeed.dat)."""
+    crds, edges, fv, fe = oracle_mod.read_cubed_seed()
+    assert np.allclose(np.linalg.norm(crds, axis=1), 1.0, atol=4e-16)
+    for f in range(6):
+        c = crds[fv[f]].mean(axis=0)
+        assert np.allclose(c / np.linalg.norm(c), crds[8 + f], atol=1e-15)
+    for o, d, l, r in edges:
+        assert o in fv[l] and d in fv[l] and o in fv[r] and d in fv[r]
