@@ -373,11 +373,11 @@ int lpm_bve_sizes(lpm_bve_ctx* ctx, int64_t* sizes4);
  * source splits (0 = automatic).  Applies to later solves and steps. */
 int lpm_bve_set_launch(lpm_bve_ctx* ctx, int targets_per_lane, int source_splits);
 
-/* FP32C: source tiles whose centre lies within `factor` tile radii of a
- * target use the explicit |y - x|^2 form, farther tiles the expanded
- * |D|^2 + |d|^2 + 2 D.d form (fewer operations; its cancellation is bounded by
- * ((factor+1)/(factor-1))^2).  Default 4; a tile holding a target's own
- * source always uses the explicit form. */
+/* FP32C and FP64 Lagrangian-scheme sums: source tiles whose centre lies
+ * within `factor` tile radii of a target use the explicit |y - x|^2 form,
+ * farther tiles the expanded (1 - x.c) - x.d form (fewer operations; its
+ * cancellation is bounded by ((factor+1)/(factor-1))^2).  Default 4; a tile
+ * holding a target's own source always uses the explicit form. */
 int lpm_bve_set_near_factor(lpm_bve_ctx* ctx, double factor);
 
 /* Stage source tiles through LDS (1, default) or read them through the

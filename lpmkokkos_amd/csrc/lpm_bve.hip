@@ -230,14 +230,15 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, do
   }
 }
 
-// One workgroup per tile: centre (mean of its sources), frame (at
-// ctr[ntiles + 3t + {0,1,2}] = n, e1, e2) and FP64 records; padding records
-// sit 8 units from the centre with q = 0.
+// One workgroup per tile: centre (mean of its sources) and radius max |d|
+// (rounded up) in ctr[t], frame (at ctr[ntiles + 3t + {0,1,2}] = n, e1, e2)
+// and FP64 records; padding records sit at the origin (offset -c) with
+// q = 0, where 1 - x.y = 1 for every target x.
 __global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ src, double4* __restrict__ ctr,
                                                           Pos3 pos, const double* __restrict__ zeta,
                                                           const double* __restrict__ area,
                                                           const int* __restrict__ smap, int ns) {
-  __shared__ double red[3][kTile / 64];
+  __shared__ double red[4][kTile / 64];
   const int t = blockIdx.x;
   const int j = t * kTile + threadIdx.x;
   const bool valid = j < ns;
@@ -264,18 +265,26 @@ __global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ sr
   double n[3], e1[3], e2[3];
   tile_frame(c, n, e1, e2);
   Src64 r;
+  double rad = 0.0;
   if (valid) {
     const double dx = p0 - c[0], dy = p1 - c[1], dz = p2 - c[2];
-    r.a = make_double4(dx, dy, dz, fma(dx, dx, fma(dy, dy, dz * dz)));
+    const double d2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    r.a = make_double4(dx, dy, dz, d2);
     r.b = make_double4(q * (p0 * n[0] + p1 * n[1] + p2 * n[2]), q * (p0 * e1[0] + p1 * e1[1] + p2 * e1[2]),
                        q * (p0 * e2[0] + p1 * e2[1] + p2 * e2[2]), q);
+    rad = sqrt(d2);
   } else {
-    r.a = make_double4(8.0, 8.0, 8.0, 192.0);
+    r.a = make_double4(-c[0], -c[1], -c[2], fma(c[0], c[0], fma(c[1], c[1], c[2] * c[2])));
     r.b = make_double4(0.0, 0.0, 0.0, 0.0);
   }
   src[j] = r;
+  rad = wave_max(rad);
+  __syncthreads();
+  if (lane == 0) red[3][wave] = rad;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    ctr[t] = make_double4(c[0], c[1], c[2], 0.0);
+    const double rr = fmax(fmax(red[3][0], red[3][1]), fmax(red[3][2], red[3][3]));
+    ctr[t] = make_double4(c[0], c[1], c[2], rr * (1.0 + 1e-6));
     double4* fr = ctr + gridDim.x + 3 * (long)t;
     fr[0] = make_double4(n[0], n[1], n[2], 0.0);
     fr[1] = make_double4(e1[0], e1[1], e1[2], 0.0);
@@ -307,7 +316,7 @@ struct PairArgs {
   int tiles_per_split;   // tiles per grid.y slice
   double* part;          // [splits][kNC][nt_pad]
   int nt_pad;
-  float near_factor;  // FP32C: near-form threshold in tile radii
+  float near_factor;  // FP32C and FP64 tile form: near-form threshold in tile radii
   // FP64 kernels: the source count on the device (may be null); tiles past
   // it hold only padding records and are skipped
   const int* ns_dev;
@@ -349,11 +358,18 @@ __device__ __forceinline__ void store_partials(const PairArgs& a, int by, const 
   }
 }
 
+// Literal denominator 1 - x.y of src/LpmBVEKernels.hpp:21-44 as three FMAs
+// from 1 (each rounding once, as the dot product's own FMAs do before the
+// exact subtraction from 1 in the four-operation form).
+__device__ __forceinline__ double dot_den(const double (&x)[3], const Src64& s) {
+  return fma(-x[2], s.a.z, fma(-x[1], s.a.y, fma(-x[0], s.a.x, 1.0)));
+}
+
 // Dot-form pair term (biotSavart and greensFn, src/LpmBVEKernels.hpp:21-44): den = 1 - x.y,
 // S += (q y)/den; psi: S3 += q ln(den).  `me` drops the target's own source.
 template <bool PSI, bool CHECK>
 __device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, bool me, double (&S)[kNC]) {
-  double den = 1.0 - (x[0] * s.a.x + x[1] * s.a.y + x[2] * s.a.z);
+  double den = dot_den(x, s);
   double q = s.a.w;
   if (CHECK && me) {
     den = 1.0;
@@ -371,9 +387,6 @@ __device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, b
 // reciprocal (rcp4_64).  The four pairs belong to one target, so a degenerate
 // pair (a target on a source that is not its own, as a divided face's centre
 // on its middle child's) leaves every other target's sums as they are.
-__device__ __forceinline__ double dot_den(const double (&x)[3], const Src64& s) {
-  return 1.0 - (x[0] * s.a.x + x[1] * s.a.y + x[2] * s.a.z);
-}
 
 template <bool PSI>
 __device__ __forceinline__ void dot_acc(double inv, double den, const Src64& s, double (&S)[kNC]) {
@@ -395,35 +408,57 @@ __device__ __forceinline__ void dot_terms4(const double (&x)[3], const Src64& s0
   dot_acc<PSI>(id, d, s3, S);
 }
 
-// Chord far form (tile-local records, see pair_block_f64) for one target and
-// four sources with one shared reciprocal: r^2 = |D|^2 + |dx|^2 + 2 D.dx,
-// F[0..2] += q y/r^2 in frame components, and with PSI F[3] += q ln(r^2 / 2).
-__device__ __forceinline__ double chord_r2(const double (&D2)[3], double E, const Src64& s) {
-  double r2 = E + s.a.w;
-  r2 = fma(D2[0], s.a.x, r2);
-  r2 = fma(D2[1], s.a.y, r2);
-  return fma(D2[2], s.a.z, r2);
+// Far form of the |y - x|^2/2 kernel (tile-local records, see pair_block_f64)
+// for one target and four sources with one shared reciprocal.  On the unit
+// sphere |y - x|^2/2 = 1 - x.y, and with y = c + d,
+//   den = A - x.d,   A = 1 - x.c  (per target and tile),
+// three FMAs per pair.  On a tile whose centre lies beyond near_factor tile
+// radii of the target, |x.d| <= A (1 + ...) is bounded by the tile radius, so
+// the cancellation in den is at most ((f + 1)/(f - 1))^2 (2.8 at f = 4);
+// nearer tiles form d = D + dx explicitly.  F[0..2] += q y/den in frame
+// components (twice q y/|y - x|^2), and with PSI F[3] += q ln(den).
+__device__ __forceinline__ double far_den64(const double (&x)[3], double A, const Src64& s) {
+  return fma(-x[2], s.a.z, fma(-x[1], s.a.y, fma(-x[0], s.a.x, A)));
 }
 
 template <bool PSI>
-__device__ __forceinline__ void chord_acc(double inv, double r2, const Src64& s, double (&F)[kNC]) {
+__device__ __forceinline__ void far_acc64(double inv, double den, const Src64& s, double (&F)[kNC]) {
   F[0] = fma(inv, s.b.x, F[0]);
   F[1] = fma(inv, s.b.y, F[1]);
   F[2] = fma(inv, s.b.z, F[2]);
-  if (PSI) F[3] = fma(s.b.w, log(0.5 * r2), F[3]);
+  if (PSI) F[3] = fma(s.b.w, log(den), F[3]);
 }
 
 template <bool PSI>
-__device__ __forceinline__ void chord_terms4(const double (&D2)[3], double E, const Src64& s0, const Src64& s1,
-                                             const Src64& s2, const Src64& s3, double (&F)[kNC]) {
-  const double a = chord_r2(D2, E, s0), b = chord_r2(D2, E, s1);
-  const double c = chord_r2(D2, E, s2), d = chord_r2(D2, E, s3);
+__device__ __forceinline__ void far_terms4_64(const double (&x)[3], double A, const Src64& s0, const Src64& s1,
+                                              const Src64& s2, const Src64& s3, double (&F)[kNC]) {
+  const double a = far_den64(x, A, s0), b = far_den64(x, A, s1);
+  const double c = far_den64(x, A, s2), d = far_den64(x, A, s3);
   double ia, ib, ic, id;
   rcp4_64(a, b, c, d, ia, ib, ic, id);
-  chord_acc<PSI>(ia, a, s0, F);
-  chord_acc<PSI>(ib, b, s1, F);
-  chord_acc<PSI>(ic, c, s2, F);
-  chord_acc<PSI>(id, d, s3, F);
+  far_acc64<PSI>(ia, a, s0, F);
+  far_acc64<PSI>(ib, b, s1, F);
+  far_acc64<PSI>(ic, c, s2, F);
+  far_acc64<PSI>(id, d, s3, F);
+}
+
+// Near form: d = D + dx formed explicitly (D = c - x), S += q d/|d|^2 in
+// global components, with PSI S3 += q ln(|d|^2/2).  `me` drops the target's
+// own source.
+template <bool PSI, bool CHECK>
+__device__ __forceinline__ void near_term64(const double (&D)[3], const Src64& s, bool me, double (&S)[kNC]) {
+  const double e0 = D[0] + s.a.x, e1 = D[1] + s.a.y, e2 = D[2] + s.a.z;
+  double r2 = fma(e0, e0, fma(e1, e1, e2 * e2));
+  double q = s.b.w;
+  if (CHECK && me) {
+    r2 = 1.0;
+    q = 0.0;
+  }
+  const double f = q * rcp64(r2);
+  S[0] = fma(f, e0, S[0]);
+  S[1] = fma(f, e1, S[1]);
+  S[2] = fma(f, e2, S[2]);
+  if (PSI) S[3] = fma(q, log(0.5 * r2), S[3]);
 }
 
 // FP64 pair sums of one target block (R targets per lane) over one source
@@ -433,11 +468,12 @@ __device__ __forceinline__ void chord_terms4(const double (&D2)[3], double E, co
 // scalar loads.
 //   FORM_DOT: dot_term for every pair.
 //   FORM_CHORD (|y - x|^2/2 form, tile-local records): with D = c - x per
-//     target and tile, tiles holding a target's own source form d = D + dx
-//     explicitly, drop the self pair and sum S += q d/|d|^2; other tiles form
-//     r^2 = |D|^2 + |dx|^2 + 2 D.dx and sum F = q y/r^2 in the tile frame,
-//     S += n F0 + e1 F1 + e2 F2.  The two differ by the component
-//     -x sum q/r^2, which the finalising cross product with x cancels.
+//     target and tile, tiles holding a target's own source or lying within
+//     near_factor tile radii of a target of the wave form d = D + dx
+//     explicitly (the self pair dropped) and sum S += q d/|d|^2; other tiles
+//     form den = (1 - x.c) - x.dx and sum F = q y/den in the tile frame,
+//     S += (n F0 + e1 F1 + e2 F2)/2.  The two differ by the component
+//     -x sum q/|d|^2, which the finalising cross product with x cancels.
 template <int R, int FORM, bool PSI, bool LDS>
 __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, const double4* __restrict__ ctr,
                                                const PairArgs& a, int bx, int by, Src64 (*sh)[kTile]) {
@@ -481,53 +517,51 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
         }
       }
     } else {
+      // c.w: the tile radius (k_prep_f64_tiles)
       const double4 c = ctr[t];
+      const double rn = (double)a.near_factor * c.w;
       double D[R][3];
+      int near = 0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         D[r][0] = c.x - x[r][0];
         D[r][1] = c.y - x[r][1];
         D[r][2] = c.z - x[r][2];
+        near |= fma(D[r][0], D[r][0], fma(D[r][1], D[r][1], D[r][2] * D[r][2])) < rn * rn;
       }
       if (has_self) {
         for (int j = 0; j < kTile; ++j) {
           const Src64 sj = ts[j];
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const double e0 = D[r][0] + sj.a.x, e1 = D[r][1] + sj.a.y, e2 = D[r][2] + sj.a.z;
-            double r2 = fma(e0, e0, fma(e1, e1, e2 * e2));
-            double q = sj.b.w;
-            if (base + j == self[r]) {
-              r2 = 1.0;
-              q = 0.0;
-            }
-            const double f = q * rcp64(r2);
-            S[r][0] = fma(f, e0, S[r][0]);
-            S[r][1] = fma(f, e1, S[r][1]);
-            S[r][2] = fma(f, e2, S[r][2]);
-            if (PSI) S[r][3] = fma(q, log(0.5 * r2), S[r][3]);
-          }
+          for (int r = 0; r < R; ++r) near_term64<PSI, true>(D[r], sj, base + j == self[r], S[r]);
+        }
+      } else if (__ballot(near) != 0) {
+#pragma unroll 2
+        for (int j = 0; j < kTile; ++j) {
+          const Src64 sj = ts[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) near_term64<PSI, false>(D[r], sj, false, S[r]);
         }
       } else {
-        double D2[R][3], E[R], F[R][kNC];
+        double A[R], F[R][kNC];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          for (int k = 0; k < 3; ++k) D2[r][k] = 2.0 * D[r][k];
-          E[r] = fma(D[r][0], D[r][0], fma(D[r][1], D[r][1], D[r][2] * D[r][2]));
+          A[r] = 1.0 - (x[r][0] * c.x + x[r][1] * c.y + x[r][2] * c.z);
           for (int k = 0; k < kNC; ++k) F[r][k] = 0.0;
         }
         for (int j = 0; j < kTile; j += 4) {
           const Src64 s0 = ts[j], s1 = ts[j + 1], s2 = ts[j + 2], s3 = ts[j + 3];
 #pragma unroll
-          for (int r = 0; r < R; ++r) chord_terms4<PSI>(D2[r], E[r], s0, s1, s2, s3, F[r]);
+          for (int r = 0; r < R; ++r) far_terms4_64<PSI>(x[r], A[r], s0, s1, s2, s3, F[r]);
         }
         const double4* fr = ctr + a.ntiles + 3 * (long)t;
         const double4 fn = fr[0], f1 = fr[1], f2 = fr[2];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          S[r][0] += fma(fn.x, F[r][0], fma(f1.x, F[r][1], f2.x * F[r][2]));
-          S[r][1] += fma(fn.y, F[r][0], fma(f1.y, F[r][1], f2.y * F[r][2]));
-          S[r][2] += fma(fn.z, F[r][0], fma(f1.z, F[r][1], f2.z * F[r][2]));
+          const double p0 = 0.5 * F[r][0], p1 = 0.5 * F[r][1], p2 = 0.5 * F[r][2];
+          S[r][0] += fma(fn.x, p0, fma(f1.x, p1, f2.x * p2));
+          S[r][1] += fma(fn.y, p0, fma(f1.y, p1, f2.y * p2));
+          S[r][2] += fma(fn.z, p0, fma(f1.z, p1, f2.z * p2));
           S[r][3] += F[r][3];
         }
       }
