@@ -465,6 +465,45 @@ def test_fp64_staging_variants(omesh, lds):
     c.close()
 
 
+@pytest.mark.parametrize("drop,near,lds", [(0, 4.0, True), (0, 2.0, False), (0, 1e30, True), (0, 0.0, True),
+                                           (137, 4.0, True), (137, 0.0, False)])
+def test_fp64_lagrangian_near_variants(omesh, drop, near, lds):
+    """The FP64 Lagrangian-scheme sums take the explicit |y - x|^2 form on tiles
+    within `near` tile radii of a target and (1 - x.c) - x.d beyond: every
+    threshold (0: the far form on every tile without a self pair; 1e30: the
+    explicit form everywhere) and a partly padded last tile (137 of level 5's
+    faces removed) agree with the oracle's chord-form sums.  Level 5, 16
+    Gaussian vortices plus a constant, so total circulation is nonzero and
+    psi's ln 2 term counts."""
+    m = omesh(5)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=6, nvort=16, sigma=0.15)
+    fz = fz[m.fmask == 0] + 0.3
+    vz = vz + 0.3
+    keep = np.ones(len(fz), bool)
+    if drop:
+        keep[np.random.default_rng(9).choice(len(fz), drop, replace=False)] = False
+        assert keep.sum() % 256 != 0
+    fx, fzk, fa = lm.fx[keep], fz[keep], lm.farea[keep]
+    fm = np.zeros(len(fx), np.uint8)
+    c = L.Context(0)
+    c.set_state(lm.vx, vz, fx, fzk, fa, fm)
+    c.set_source_staging(lds)
+    c.set_near_factor(near)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR, want_psi=True)
+    s = c.get_state()
+    ov = O.velocity_chord(lm.vx, fx, fzk, fa, fm, False)
+    of = O.velocity_chord(fx, fx, fzk, fa, fm, True)
+    pv = O.streamfn_chord(lm.vx, fx, fzk, fa, fm, False)
+    pf = O.streamfn_chord(fx, fx, fzk, fa, fm, True)
+    # far pairs of adjacent tiles (near = 0) lose up to ~1e-16 / (h^2 / 2) relative
+    tol = 1e-12 if near == 0.0 else 1e-13
+    ev = max(normwise(s["vu"], ov), normwise(s["fu"], of))
+    ep = max(normwise(s["vpsi"], pv), normwise(s["fpsi"], pf))
+    assert ev < tol and ep < tol, (ev, ep)
+    c.close()
+
+
 @pytest.mark.parametrize("level", range(6))
 def test_gpu_reproduces_published_poisson_table(omesh, level):
     """SpherePoisson's Y_5^4 problem (src/LpmSpherePoisson.hpp:428-475) through
