@@ -280,6 +280,12 @@ typedef struct lpm_ic_params {
  * rounding, not bit for bit).  Positions, numbering and areas are those of
  * lpm_icos_tree_init(level, NULL, ...); nf counts every tree face. */
 int lpm_bve_set_state_level(lpm_bve_ctx* ctx, int level, const lpm_ic_params* ic);
+/* The same for MeshSeed<SeedType> of either seed (lpm_seed_kind): the
+ * cubed-sphere tree of CubedSphereSeed (src/LpmMeshSeed.hpp:111) with
+ * lpm_seed_tree_init's positions, numbering and areas, cached per (device,
+ * seed, level).  lpm_bve_set_state_level(ctx, level, ic) is this call with
+ * LPM_SEED_ICOS_TRI. */
+int lpm_bve_set_state_seed_level(lpm_bve_ctx* ctx, int seed_kind, int level, const lpm_ic_params* ic);
 
 /* Free the cached meshes of `device` (all devices if device < 0) that no
  * context uses; *kept (may be NULL) receives the number still in use.  No

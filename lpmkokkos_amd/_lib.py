@@ -78,6 +78,7 @@ SIGNATURES = {
     "lpm_bve_destroy": [C.c_void_p],
     "lpm_bve_set_state": [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _bp],
     "lpm_bve_set_state_level": [C.c_void_p, C.c_int, C.POINTER(IcParams)],
+    "lpm_bve_set_state_seed_level": [C.c_void_p, C.c_int, C.c_int, C.POINTER(IcParams)],
     "lpm_mesh_cache_clear": [C.c_int, C.POINTER(C.c_int)],
     "lpm_bve_set_velocity": [C.c_void_p, _dp, _dp],
     "lpm_bve_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int],

@@ -316,8 +316,10 @@ class Context:
              np.ascontiguousarray(fzeta, np.float64), np.ascontiguousarray(farea, np.float64),
              np.ascontiguousarray(fmask, np.uint8))
 
-    def set_state_level(self, level: int, ic: "VorticityInitialCondition", subtract_mean: bool = False):
-        """State of the level-`level` icosahedral tree (every tree face) from the
+    def set_state_level(self, level: int, ic: "VorticityInitialCondition", subtract_mean: bool = False,
+                        seed_kind: int = _lib.SEED_ICOS_TRI):
+        """State of the level-`level` tree of seed `seed_kind` (every tree face;
+        icosahedral by default, SEED_CUBED_SPHERE for the cubed sphere) from the
         device-resident mesh cache, with `ic` evaluated on the device."""
         nvort = 0 if ic.amps is None else len(ic.amps)
         centers = np.ascontiguousarray(ic.centers, np.float64) if nvort else None
@@ -325,7 +327,7 @@ class Context:
         p = _lib.IcParams(float(ic.sbr), int(ic.rh_n), float(ic.rh_w), float(ic.rh_K), nvort,
                           centers.ctypes.data if nvort else None, amps.ctypes.data if nvort else None,
                           float(ic.sigma), int(subtract_mean))
-        call("lpm_bve_set_state_level", self.h, int(level), C.byref(p))
+        call("lpm_bve_set_state_seed_level", self.h, int(seed_kind), int(level), C.byref(p))
         sizes = self.sizes()
         self.nv, self.nf = sizes[0], sizes[1]
 

@@ -1262,10 +1262,10 @@ __global__ __launch_bounds__(kBlock) void k_sub_mean(double* __restrict__ zeta, 
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
-// device-resident leaf meshes, one per (device, level), shared by contexts
+// device-resident leaf meshes, one per (device, seed, level), shared by contexts
 // ---------------------------------------------------------------------------
 struct MeshEntry {
-  int device = 0, level = 0;
+  int device = 0, kind = 0, level = 0;
   int64_t nv = 0, nf = 0;
   int ns = 0, nt = 0;
   std::vector<int64_t> fmap;        // active face -> tree face
@@ -1764,7 +1764,7 @@ namespace lpm {
 namespace {
 
 std::mutex g_mesh_mu;
-std::map<std::pair<int, int>, MeshEntry*> g_meshes;
+std::map<std::tuple<int, int, int>, MeshEntry*> g_meshes;  // (device, seed kind, level)
 
 void release_mesh(lpm_bve_ctx* c) {
   c->face_zeta_pending = false;
@@ -1835,11 +1835,11 @@ int ctx_alloc_state(lpm_bve_ctx* c, bool own_static) {
   return LPM_OK;
 }
 
-// The cached level-`level` mesh on `device`, built and uploaded on first use;
-// counts one more user.
-int get_mesh(int device, int level, MeshEntry** out) {
+// The cached level-`level` tree of seed `kind` on `device`, built and
+// uploaded on first use; counts one more user.
+int get_mesh(int device, int kind, int level, MeshEntry** out) {
   std::lock_guard<std::mutex> g(g_mesh_mu);
-  auto key = std::make_pair(device, level);
+  const auto key = std::make_tuple(device, kind, level);
   auto it = g_meshes.find(key);
   if (it != g_meshes.end()) {
     ++it->second->users;
@@ -1848,9 +1848,10 @@ int get_mesh(int device, int level, MeshEntry** out) {
   }
   MeshEntry* m = new MeshEntry();
   m->device = device;
+  m->kind = kind;
   m->level = level;
   try {
-    const TreeMesh t = build_icos_tree(level, builtin_icos_seed());
+    const TreeMesh t = build_tree(kind, level);
     m->nv = t.nverts();
     m->nf = t.nfaces();
     m->fmask = t.face_mask;
@@ -2276,8 +2277,17 @@ int lpm_bve_set_state(lpm_bve_ctx* c, int64_t nv, const double* vx, const double
   return ok();
 }
 
+static_assert(LPM_SEED_ICOS_TRI == lpm::kSeedIcosTri && LPM_SEED_CUBED_SPHERE == lpm::kSeedCubedSphere,
+              "seed kinds of the C ABI and the mesh builder agree");
+
 int lpm_bve_set_state_level(lpm_bve_ctx* c, int level, const lpm_ic_params* ic) {
+  return lpm_bve_set_state_seed_level(c, LPM_SEED_ICOS_TRI, level, ic);
+}
+
+int lpm_bve_set_state_seed_level(lpm_bve_ctx* c, int seed_kind, int level, const lpm_ic_params* ic) {
   if (!c || !ic) return fail(LPM_ERR_ARG, "lpm_bve_set_state_level: null argument");
+  if (seed_kind != LPM_SEED_ICOS_TRI && seed_kind != LPM_SEED_CUBED_SPHERE)
+    return fail(LPM_ERR_ARG, "lpm_bve_set_state_level: unknown seed kind");
   if (level < 0 || level > 10) return fail(LPM_ERR_ARG, "lpm_bve_set_state_level: level must be in [0, 10]");
   if (ic->nvort < 0 || (ic->nvort > 0 && (!ic->centers || !ic->amps)) || !(ic->sigma > 0.0))
     return fail(LPM_ERR_ARG, "lpm_bve_set_state_level: bad field parameters");
@@ -2286,7 +2296,7 @@ int lpm_bve_set_state_level(lpm_bve_ctx* c, int level, const lpm_ic_params* ic) 
   free_ctx_memory(c);
   c->eb = EvalBuffers();
   MeshEntry* m = nullptr;
-  int rc = get_mesh(c->device, level, &m);
+  int rc = get_mesh(c->device, seed_kind, level, &m);
   if (rc) return rc;
   c->mesh = m;
   c->nv = m->nv;

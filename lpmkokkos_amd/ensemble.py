@@ -14,8 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import PREC_FP32C, SCHEME_LAGR, call
-from .bve import Context, TreeMesh, VorticityInitialCondition, area_mean
+from ._lib import PREC_FP32C, SCHEME_LAGR, SEED_CUBED_SPHERE, SEED_ICOS_TRI, call
+from .bve import Context, CubedSphereSeed, MeshSeed, TreeMesh, VorticityInitialCondition, area_mean
 
 
 def rng_draws(seed: int, nuniform: int = 0, nnormal: int = 0):
@@ -104,9 +104,11 @@ def state_bytes(nv: int, nactive: int) -> int:
 
 @dataclass
 class MemberSpec:
-    """One ensemble member or forecast request: mesh level and initial field
-    (RH wave of wavenumber rh_n with K = 0.1 * k_factor, w = 0.1, plus nvort
-    Gaussian vortices of amplitude N(0, amp) and width sigma, mean removed)."""
+    """One ensemble member or forecast request: mesh (level of the icosahedral
+    tree, or of the cubed-sphere tree with seed = SEED_CUBED_SPHERE) and
+    initial field (RH wave of wavenumber rh_n with K = 0.1 * k_factor, w = 0.1,
+    plus nvort Gaussian vortices of amplitude N(0, amp) and width sigma, mean
+    removed)."""
     level: int
     k_factor: float = 1.0
     rh_n: int = 4
@@ -114,16 +116,18 @@ class MemberSpec:
     amp: float = 0.05
     sigma: float = 0.15
     vortex_seed: int = 0
+    seed: int = SEED_ICOS_TRI
 
 
-_MESHES: dict[int, TreeMesh] = {}
+_MESHES: dict[tuple[int, int], TreeMesh] = {}
 
 
-def leaf_mesh(level: int):
-    """Vertices and active faces of a level-`level` mesh (cached per level)."""
-    if level not in _MESHES:
-        _MESHES[level] = TreeMesh(level)
-    m = _MESHES[level]
+def leaf_mesh(level: int, seed: int = SEED_ICOS_TRI):
+    """Vertices and active faces of a level-`level` tree of seed `seed`
+    (cached per seed and level)."""
+    if (seed, level) not in _MESHES:
+        _MESHES[(seed, level)] = TreeMesh(level, MeshSeed(CubedSphereSeed) if seed == SEED_CUBED_SPHERE else None)
+    m = _MESHES[(seed, level)]
     lv = m.fmask == 0
     return m, np.ascontiguousarray(m.fx[lv]), np.ascontiguousarray(m.farea[lv])
 
@@ -138,7 +142,7 @@ def member_ic(spec: MemberSpec) -> VorticityInitialCondition:
 
 def member_state(spec: MemberSpec):
     """Host-built state (leaf faces only), mean removed."""
-    m, fx, farea = leaf_mesh(spec.level)
+    m, fx, farea = leaf_mesh(spec.level, spec.seed)
     ic = member_ic(spec)
     vz = ic.eval_many(m.vx)
     fz = ic.eval_many(fx)
@@ -152,7 +156,7 @@ def make_member(spec: MemberSpec, device: int = 0, on_device: bool = True, strea
     stream: queue the context's work there (e.g. its ensemble's stream)."""
     c = Context(device, stream=stream)
     if on_device:
-        c.set_state_level(spec.level, member_ic(spec), subtract_mean=True)
+        c.set_state_level(spec.level, member_ic(spec), subtract_mean=True, seed_kind=spec.seed)
         return c
     vx, vz, fx, fz, farea = member_state(spec)
     c.set_state(vx, vz, fx, fz, farea, np.zeros(len(fz), np.uint8))

@@ -45,7 +45,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import PREC_FP32C, SCHEME_LAGR
+from ._lib import PREC_FP32C, SCHEME_LAGR, SEED_CUBED_SPHERE
 from .bve import Context, VorticityInitialCondition
 from .ensemble import Ensemble, MemberSpec, make_member, rng_draws, state_bytes
 
@@ -120,8 +120,11 @@ def load_manifest(directory: str) -> list[Request]:
 
 
 def request_bytes(spec: MemberSpec) -> int:
-    nv = 2 + 10 * 4 ** spec.level
-    return state_bytes(nv, 20 * 4 ** spec.level)
+    # vertices and leaves of the level-L tree: 2 + 10 4^L and 20 4^L
+    # (icosahedral), 2 + 6 4^L and 6 4^L (cubed sphere)
+    if spec.seed == SEED_CUBED_SPHERE:
+        return state_bytes(2 + 6 * 4 ** spec.level, 6 * 4 ** spec.level)
+    return state_bytes(2 + 10 * 4 ** spec.level, 20 * 4 ** spec.level)
 
 
 class _Lane:
@@ -192,11 +195,12 @@ class Server:
     def lane_for(self, level: int) -> _Lane:
         return next(lane for lane in self.lanes if lane.lo <= level <= lane.hi)
 
-    def warm(self, levels):
-        """Build the device mesh cache for `levels` (once per process)."""
-        for lev in sorted(set(levels)):
+    def warm(self, meshes):
+        """Build the device mesh cache for the (seed, level) pairs `meshes`
+        (once per process)."""
+        for seed, lev in sorted(set(meshes)):
             c = Context(self.device, stream=self.lanes[0].stream)
-            c.set_state_level(lev, make_ic_zero())
+            c.set_state_level(lev, make_ic_zero(), seed_kind=seed)
             c.close()
 
     def free_lane(self) -> int | None:
@@ -294,7 +298,7 @@ class Server:
             if r.spec.level >= self.large_level and not self.lane_streams:
                 raise ValueError(f"request {r.rid} (level {r.spec.level}) needs a large lane and the server has none "
                                  f"(large_lanes=0, large_level={self.large_level})")
-        self.warm(r.spec.level for r in requests)
+        self.warm((r.spec.seed, r.spec.level) for r in requests)
         pending = sorted(requests, key=lambda r: r.arrival)
         done: list[Request] = []
         t0 = time.perf_counter()

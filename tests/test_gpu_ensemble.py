@@ -190,3 +190,24 @@ def test_batching_configuration_all_members(omesh):
     assert checked == {3, 4, 5, 6, 7}
     for c in mixed:
         c.close()
+
+
+def test_mixed_seed_members_match_alone():
+    """Icosahedral and cubed-sphere members (built on the device from their
+    cached trees) in one ensemble: each bit-identical to its run alone."""
+    sp = [E.MemberSpec(level=3, vortex_seed=1), E.MemberSpec(level=4, vortex_seed=2, seed=L.SEED_CUBED_SPHERE),
+          E.MemberSpec(level=5, vortex_seed=3, seed=L.SEED_CUBED_SPHERE), E.MemberSpec(level=4, vortex_seed=4)]
+    alone = [E.make_member(s) for s in sp]
+    mixed = [E.make_member(s) for s in sp]
+    assert mixed[1].sizes()[1] == L.TreeMesh(4, L.MeshSeed(L.CubedSphereSeed)).nfaces
+    ens = E.Ensemble(0)
+    for c in mixed:
+        ens.add(c)
+    for c in alone:
+        c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=3)
+    ens.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=3)
+    for a, b in zip(alone, mixed):
+        assert same(a, b)
+    ens.close()
+    for c in alone + mixed:
+        c.close()

@@ -134,3 +134,59 @@ def test_state_level_errors():
     with pytest.raises(L.LpmError):
         c.set_state_level(2, bad)
     c.close()
+
+
+@pytest.mark.parametrize("level", [0, 3, 5])
+def test_cubed_state_level_matches_tree_init_and_host_field(level):
+    """lpm_bve_set_state_seed_level for CubedSphereSeed: the cached cubed-sphere
+    tree is lpm_seed_tree_init's bit for bit, the field the host evaluation's
+    to rounding, and the icosahedral tree of the same level is cached apart."""
+    m = L.TreeMesh(level, L.MeshSeed(L.CubedSphereSeed))
+    ic = field()
+    c = L.Context(0)
+    c.set_state_level(level, ic, seed_kind=L.SEED_CUBED_SPHERE)
+    assert c.sizes()[:2] == (m.nverts, m.nfaces)
+    s = c.get_state()
+    assert np.array_equal(s["vx"], m.vx) and np.array_equal(s["fx"], m.fx)
+    hv, hf = ic.eval_many(m.vx), ic.eval_many(m.fx)
+    lv = m.fmask == 0
+    scale = max(np.abs(hv).max(), np.abs(hf).max())
+    assert np.abs(s["vzeta"] - hv).max() < 1e-14 * scale
+    assert np.abs(s["fzeta"][lv] - hf[lv]).max() < 1e-14 * scale
+    d = L.Context(0)
+    d.set_state_level(level, ic)
+    assert d.sizes()[:2] == (L.TreeMesh(level).nverts, L.TreeMesh(level).nfaces)
+    c.close()
+    d.close()
+
+
+def test_cubed_state_level_steps_like_host_state():
+    """Five FP64 Lagrangian steps from the device-built level-4 cubed-sphere
+    state against five from the host-built one."""
+    level = 4
+    m = L.TreeMesh(level, L.MeshSeed(L.CubedSphereSeed))
+    ic = field()
+    a = L.Context(0)
+    a.set_state_level(level, ic, subtract_mean=True, seed_kind=L.SEED_CUBED_SPHERE)
+    lv = m.fmask == 0
+    hv, hf = ic.eval_many(m.vx), ic.eval_many(m.fx)
+    mean = L.area_mean(hf[lv], m.farea[lv], np.zeros(int(lv.sum()), np.uint8))
+    b = L.Context(0)
+    b.set_state(m.vx, hv - mean, m.fx, hf - mean, m.farea, m.fmask)
+    for c in (a, b):
+        c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP64, nsteps=5)
+    sa, sb = a.get_state(), b.get_state()
+    assert normwise(sa["vx"], sb["vx"]) < 1e-12 and normwise(sa["fx"][lv], sb["fx"][lv]) < 1e-12
+    assert normwise(sa["vu"], sb["vu"]) < 1e-12 and normwise(sa["fu"][lv], sb["fu"][lv]) < 1e-12
+    a.close()
+    b.close()
+
+
+def test_state_seed_level_errors():
+    c = L.Context(0)
+    p = _lib.IcParams(0.0, 4, 0.1, 0.1, 0, None, None, 1.0, 0)
+    with pytest.raises(L.LpmError):
+        _lib.call("lpm_bve_set_state_seed_level", c.h, 7, 3, _lib.C.byref(p))
+    with pytest.raises(L.LpmError):
+        c.set_state_level(11, field(), seed_kind=L.SEED_CUBED_SPHERE)
+    c.close()
