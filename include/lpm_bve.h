@@ -302,6 +302,12 @@ int lpm_bve_set_velocity(lpm_bve_ctx* ctx, const double* vu, const double* fu);
  * solves of BVESphere::init_vorticity (src/LpmBVESphere.cpp:151-168).
  * scheme selects the kernel denominator (REF: 1 - x.y; LAGR: |y-x|^2/2). */
 int lpm_bve_solve(lpm_bve_ctx* ctx, int prec, int scheme, int want_psi);
+/* Stream function alone at the current state: the reference driver's
+ * post-timestep BVEVertexStreamFn / BVEFaceStreamFn (tests/LpmBVETest.cpp:155-159,
+ * src/LpmBVEKernels.hpp:140-161, 263-283).  The velocity the context holds
+ * (the next step's entry velocity) is left as it is, so steps after it are
+ * the steps without it, bit for bit. */
+int lpm_bve_streamfn(lpm_bve_ctx* ctx, int prec, int scheme);
 
 /* `nsteps` calls of BVERK4::advance_timestep (src/LpmBVERK4_Impl.hpp:42-130)
  * on the resident state.  On return the velocity is valid at the new state.

@@ -284,9 +284,10 @@ class BVESphere {
   }
 
   // Stream function at the current state (the post-step solve of
-  // tests/LpmBVETest.cpp:155-159).
+  // tests/LpmBVETest.cpp:155-159); the velocity, the next step's entry
+  // velocity, is left as it is.
   void solve_streamfn(int prec = LPM_PREC_FP64, int scheme = LPM_SCHEME_REF) {
-    check(lpm_bve_solve(ctx_, prec, scheme, 1), "lpm_bve_solve");
+    check(lpm_bve_streamfn(ctx_, prec, scheme), "lpm_bve_streamfn");
   }
 
   void updateHost() {

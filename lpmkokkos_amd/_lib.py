@@ -82,6 +82,7 @@ SIGNATURES = {
     "lpm_mesh_cache_clear": [C.c_int, C.POINTER(C.c_int)],
     "lpm_bve_set_velocity": [C.c_void_p, _dp, _dp],
     "lpm_bve_solve": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+    "lpm_bve_streamfn": [C.c_void_p, C.c_int, C.c_int],
     "lpm_bve_advance": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
     "lpm_bve_get_state": [C.c_void_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
     "lpm_bve_checkpoint": [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_double],

@@ -338,6 +338,11 @@ class Context:
     def solve(self, prec: int = PREC_FP64, scheme: int = SCHEME_REF, want_psi: bool = False):
         call("lpm_bve_solve", self.h, prec, scheme, int(want_psi))
 
+    def streamfn(self, prec: int = PREC_FP64, scheme: int = SCHEME_REF):
+        """Stream function alone at the current state (the reference driver's
+        post-timestep solve); the held velocity is left as it is."""
+        call("lpm_bve_streamfn", self.h, prec, scheme)
+
     def advance(self, dt: float, omega: float, scheme: int = SCHEME_REF, prec: int = PREC_FP64, nsteps: int = 1,
                 use_graph: bool = True):
         call("lpm_bve_advance", self.h, dt, omega, scheme, prec, nsteps, int(use_graph))
@@ -510,6 +515,11 @@ class BVESphere:
         self.host["absVortFaces"] = fz + 2 * self.Omega * m.fx[:, 2]
         self.ctx.set_state(m.vx, vz, m.fx, fz, m.farea, m.fmask)
         self.ctx.solve(prec, scheme, want_psi=True)
+
+    def solve_streamfn(self, prec: int = PREC_FP64, scheme: int = SCHEME_REF):
+        """Stream function at the current state (tests/LpmBVETest.cpp:155-159);
+        the velocity, the next step's entry velocity, is left as it is."""
+        self.ctx.streamfn(prec, scheme)
 
     def updateHost(self):
         s = self.ctx.get_state()

@@ -182,6 +182,9 @@ int run(Options opt) {
                               sphere.faces.mask, n);
       steps_done += n;
       if (output_interval > 0) {
+        // the post-timestep stream-function solve (tests/LpmBVETest.cpp:155-159),
+        // here at output steps: it leaves the entry velocity of the next step alone
+        sphere.solve_streamfn(prec, scheme);
         sphere.updateHost();
         std::printf("\tstep %d, t = %g, circulation = %.3e\n", steps_done, sphere.t, sphere.circulation());
         write_output(steps_done);

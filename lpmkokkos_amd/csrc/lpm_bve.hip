@@ -2414,6 +2414,25 @@ int lpm_bve_set_velocity(lpm_bve_ctx* c, const double* vu, const double* fu) {
   return ok();
 }
 
+int lpm_bve_streamfn(lpm_bve_ctx* c, int prec, int scheme) {
+  if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_streamfn: no state");
+  if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32C) return fail(LPM_ERR_ARG, "lpm_bve_streamfn: bad precision");
+  if (scheme != LPM_SCHEME_REF && scheme != LPM_SCHEME_LAGR) return fail(LPM_ERR_ARG, "lpm_bve_streamfn: bad scheme");
+  LPM_HIP(hipSetDevice(c->device));
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  EvalRequest q;
+  int rc = ctx_request(c, c->x, c->zeta, prec, form, true, q);
+  if (rc) return rc;
+  // the pair kernels sum velocity and stream function together; the
+  // velocity goes to the fourth position increment (free between steps), so
+  // the entry velocity of the next step stays the one the last step left
+  q.u = MutPos3{c->kx[3], 1, (long)c->nt};
+  if ((rc = enqueue_eval(q, c->eb, c->stream, nullptr, nullptr))) return rc;
+  c->psi_valid = true;
+  if ((rc = ctx_mark_done(c))) return rc;
+  return ok();
+}
+
 int lpm_bve_solve(lpm_bve_ctx* c, int prec, int scheme, int want_psi) {
   if (!c || !c->x) return fail(LPM_ERR_STATE, "lpm_bve_solve: no state");
   if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32C) return fail(LPM_ERR_ARG, "lpm_bve_solve: bad precision");

@@ -376,6 +376,11 @@ def test_cpp_driver_writes_the_reference_files(tmp_path):
     assert np.all(d0["point"]["vertex_position_error"] == 0) and np.abs(d0["cell"]["face_velocity_error"]).max() < 0.1
     d2 = read_xml(str(tmp_path / names[2]))
     assert 0 < np.abs(d2["cell"]["face_position_error"]).max() < 0.01
+    # the stream function of each output step is solved at that step (the
+    # reference's post-timestep solve): for solid-body rotation about z it is a
+    # function of z, which each particle keeps, so every output has the first's
+    p0, p2 = d0["point"]["stream_fn"].ravel(), d2["point"]["stream_fn"].ravel()
+    assert np.abs(p2).max() > 0.1 and np.abs(p2 - p0).max() < 1e-3 * np.abs(p0).max()
     # the driver's default seed, the cubed sphere: "quad" files of quadrilaterals
     qdir = tmp_path / "quad"
     qdir.mkdir()

@@ -610,3 +610,33 @@ def test_cubed_sphere_bve_matches_oracle(tmp_path):
     assert normwise(s32["vu"], ov) < 1e-6 and normwise(s32["fu"][lv], of[lv]) < 1e-6
     c.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec,scheme,tol", [(L.PREC_FP64, L.SCHEME_REF, 1e-13), (L.PREC_FP32C, L.SCHEME_LAGR, 1e-6)])
+def test_streamfn_alone_leaves_the_steps_alone(omesh, prec, scheme, tol):
+    """lpm_bve_streamfn (the reference driver's post-timestep stream-function
+    solve, tests/LpmBVETest.cpp:155-159) between steps: psi matches the oracle
+    at the current state, the held velocity is untouched, and the steps after
+    it are bit for bit those of a run without it."""
+    m = omesh(4)
+    lm = LeafMesh(m)
+    vz, fz = rh_field(lm.vx), rh_field(lm.fx)
+    a, b = make_ctx(lm, vz, fz), make_ctx(lm, vz, fz)
+    for c in (a, b):
+        c.advance(0.01, TWO_PI, scheme, prec, nsteps=2)
+    before = a.get_state()
+    a.streamfn(prec, scheme)
+    s = a.get_state()
+    for k in ("vx", "vu", "fx", "fu", "fzeta"):
+        assert np.array_equal(s[k], before[k]), k
+    fn = O.streamfn if scheme == L.SCHEME_REF else O.streamfn_chord
+    pv = fn(s["vx"], s["fx"], s["fzeta"], lm.farea, lm.fmask, False)
+    pf = fn(s["fx"], s["fx"], s["fzeta"], lm.farea, lm.fmask, True)
+    assert normwise(s["vpsi"], pv) < tol and normwise(s["fpsi"], pf) < tol
+    for c in (a, b):
+        c.advance(0.01, TWO_PI, scheme, prec, nsteps=2)
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
