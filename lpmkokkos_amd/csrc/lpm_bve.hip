@@ -358,6 +358,58 @@ __device__ __forceinline__ void store_partials(const PairArgs& a, int by, const 
   }
 }
 
+// Natural logarithm of a pair denominator for the stream-function sums, by
+// the table-driven reduction of P. T. P. Tang ("Table-driven implementation
+// of the logarithm function in IEEE floating-point arithmetic", ACM TOMS 16,
+// 1990) with a 129-entry table: x = 2^k m, m in [1, 2); F = 1 + j/128 with j
+// the nearest multiple of 1/128 to m - 1 (j = 0..128), f = m - F (exact, in
+// [-2^-8, 2^-8]);
+//   log x = (k ln 2 + log F) + log1p(f/F),
+// log1p(r) by its Taylor series to r^7 (remainder below r^8/8 < 7e-21); at
+// j = 128 (x just below a power of two) k ln 2 + log 2 cancels exactly.  The
+// table {log F, 1/F} is built per workgroup with the library's log
+// (log_table_fill).  About 20 operations per call against some 75 for the
+// library's log; measured against long-double logarithms (20M arguments
+// over [1e-11, 16)): within 1.5 units in the last place where |log x| > 1,
+// and within 1.9e-16 absolute where |log x| <= 1 (near x = 1 from below the
+// sum k ln 2 + log F keeps one rounding of ln 2's size).  Zero, negative,
+// subnormal and non-finite arguments go to the library (the reference's
+// literal 1 - x.y of an off-sphere stage argument can be <= 0).
+constexpr int kLogTable = 129;
+
+__device__ __forceinline__ void log_table_fill(double2* tab) {
+  if (threadIdx.x < kLogTable) {
+    const double F = 1.0 + threadIdx.x * (1.0 / 128);
+    tab[threadIdx.x] = make_double2(log(F), 1.0 / F);
+  }
+}
+
+// The library's log for plog's rare arguments, out of line so that the pair
+// loops do not carry a copy of it per pair.
+__device__ __attribute__((noinline)) double log_special(double x) { return log(x); }
+
+__device__ __forceinline__ double plog(double x, const double2* __restrict__ tab) {
+  // anything but a positive normal finite number (class bits: +normal = 256)
+  if (!__builtin_amdgcn_class(x, 256)) return log_special(x);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned hi = (unsigned)(b >> 32);
+  const int k = (int)(hi >> 20) - 1023;
+  const int j = (int)(((hi >> 12) & 255) + 1) >> 1;
+  const double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+  // F = 1 + j/128 (j = 128: 2) from its bits
+  const double F = __longlong_as_double((long long)((unsigned long long)(0x3FF00000u + ((unsigned)j << 13)) << 32));
+  const double2 t = tab[j];
+  const double r = (m - F) * t.y;
+  double p = 1.0 / 7;
+  p = fma(p, r, -1.0 / 6);
+  p = fma(p, r, 1.0 / 5);
+  p = fma(p, r, -1.0 / 4);
+  p = fma(p, r, 1.0 / 3);
+  p = fma(p, r, -1.0 / 2);
+  p = fma(p, r, 1.0);
+  return fma(p, r, fma((double)k, kLn2, t.x));
+}
+
 // Literal denominator 1 - x.y of src/LpmBVEKernels.hpp:21-44 as three FMAs
 // from 1 (each rounding once, as the dot product's own FMAs do before the
 // exact subtraction from 1 in the four-operation form).
@@ -368,7 +420,8 @@ __device__ __forceinline__ double dot_den(const double (&x)[3], const Src64& s) 
 // Dot-form pair term (biotSavart and greensFn, src/LpmBVEKernels.hpp:21-44): den = 1 - x.y,
 // S += (q y)/den; psi: S3 += q ln(den).  `me` drops the target's own source.
 template <bool PSI, bool CHECK>
-__device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, bool me, double (&S)[kNC]) {
+__device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, bool me, double (&S)[kNC],
+                                         const double2* lt) {
   double den = dot_den(x, s);
   double q = s.a.w;
   if (CHECK && me) {
@@ -380,7 +433,7 @@ __device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, b
   S[0] = fma(inv, s.b.x, S[0]);
   S[1] = fma(inv, s.b.y, S[1]);
   S[2] = fma(inv, s.b.z, S[2]);
-  if (PSI) S[3] = fma(q, log(den), S[3]);
+  if (PSI) S[3] = fma(q, plog(den, lt), S[3]);
 }
 
 // dot_term's arithmetic for one target and four sources with one shared
@@ -389,23 +442,24 @@ __device__ __forceinline__ void dot_term(const double (&x)[3], const Src64& s, b
 // on its middle child's) leaves every other target's sums as they are.
 
 template <bool PSI>
-__device__ __forceinline__ void dot_acc(double inv, double den, const Src64& s, double (&S)[kNC]) {
+__device__ __forceinline__ void dot_acc(double inv, double den, const Src64& s, double (&S)[kNC],
+                                        const double2* lt) {
   S[0] = fma(inv, s.b.x, S[0]);
   S[1] = fma(inv, s.b.y, S[1]);
   S[2] = fma(inv, s.b.z, S[2]);
-  if (PSI) S[3] = fma(s.a.w, log(den), S[3]);
+  if (PSI) S[3] = fma(s.a.w, plog(den, lt), S[3]);
 }
 
 template <bool PSI>
 __device__ __forceinline__ void dot_terms4(const double (&x)[3], const Src64& s0, const Src64& s1,
-                                           const Src64& s2, const Src64& s3, double (&S)[kNC]) {
+                                           const Src64& s2, const Src64& s3, double (&S)[kNC], const double2* lt) {
   const double a = dot_den(x, s0), b = dot_den(x, s1), c = dot_den(x, s2), d = dot_den(x, s3);
   double ia, ib, ic, id;
   rcp4_64(a, b, c, d, ia, ib, ic, id);
-  dot_acc<PSI>(ia, a, s0, S);
-  dot_acc<PSI>(ib, b, s1, S);
-  dot_acc<PSI>(ic, c, s2, S);
-  dot_acc<PSI>(id, d, s3, S);
+  dot_acc<PSI>(ia, a, s0, S, lt);
+  dot_acc<PSI>(ib, b, s1, S, lt);
+  dot_acc<PSI>(ic, c, s2, S, lt);
+  dot_acc<PSI>(id, d, s3, S, lt);
 }
 
 // Far form of the |y - x|^2/2 kernel (tile-local records, see pair_block_f64)
@@ -422,31 +476,34 @@ __device__ __forceinline__ double far_den64(const double (&x)[3], double A, cons
 }
 
 template <bool PSI>
-__device__ __forceinline__ void far_acc64(double inv, double den, const Src64& s, double (&F)[kNC]) {
+__device__ __forceinline__ void far_acc64(double inv, double den, const Src64& s, double (&F)[kNC],
+                                          const double2* lt) {
   F[0] = fma(inv, s.b.x, F[0]);
   F[1] = fma(inv, s.b.y, F[1]);
   F[2] = fma(inv, s.b.z, F[2]);
-  if (PSI) F[3] = fma(s.b.w, log(den), F[3]);
+  if (PSI) F[3] = fma(s.b.w, plog(den, lt), F[3]);
 }
 
 template <bool PSI>
 __device__ __forceinline__ void far_terms4_64(const double (&x)[3], double A, const Src64& s0, const Src64& s1,
-                                              const Src64& s2, const Src64& s3, double (&F)[kNC]) {
+                                              const Src64& s2, const Src64& s3, double (&F)[kNC],
+                                              const double2* lt) {
   const double a = far_den64(x, A, s0), b = far_den64(x, A, s1);
   const double c = far_den64(x, A, s2), d = far_den64(x, A, s3);
   double ia, ib, ic, id;
   rcp4_64(a, b, c, d, ia, ib, ic, id);
-  far_acc64<PSI>(ia, a, s0, F);
-  far_acc64<PSI>(ib, b, s1, F);
-  far_acc64<PSI>(ic, c, s2, F);
-  far_acc64<PSI>(id, d, s3, F);
+  far_acc64<PSI>(ia, a, s0, F, lt);
+  far_acc64<PSI>(ib, b, s1, F, lt);
+  far_acc64<PSI>(ic, c, s2, F, lt);
+  far_acc64<PSI>(id, d, s3, F, lt);
 }
 
 // Near form: d = D + dx formed explicitly (D = c - x), S += q d/|d|^2 in
 // global components, with PSI S3 += q ln(|d|^2/2).  `me` drops the target's
 // own source.
 template <bool PSI, bool CHECK>
-__device__ __forceinline__ void near_term64(const double (&D)[3], const Src64& s, bool me, double (&S)[kNC]) {
+__device__ __forceinline__ void near_term64(const double (&D)[3], const Src64& s, bool me, double (&S)[kNC],
+                                            const double2* lt) {
   const double e0 = D[0] + s.a.x, e1 = D[1] + s.a.y, e2 = D[2] + s.a.z;
   double r2 = fma(e0, e0, fma(e1, e1, e2 * e2));
   double q = s.b.w;
@@ -458,7 +515,7 @@ __device__ __forceinline__ void near_term64(const double (&D)[3], const Src64& s
   S[0] = fma(f, e0, S[0]);
   S[1] = fma(f, e1, S[1]);
   S[2] = fma(f, e2, S[2]);
-  if (PSI) S[3] = fma(q, log(0.5 * r2), S[3]);
+  if (PSI) S[3] = fma(q, plog(0.5 * r2, lt), S[3]);
 }
 
 // FP64 pair sums of one target block (R targets per lane) over one source
@@ -476,7 +533,8 @@ __device__ __forceinline__ void near_term64(const double (&D)[3], const Src64& s
 //     -x sum q/|d|^2, which the finalising cross product with x cancels.
 template <int R, int FORM, bool PSI, bool LDS>
 __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, const double4* __restrict__ ctr,
-                                               const PairArgs& a, int bx, int by, Src64 (*sh)[kTile]) {
+                                               const PairArgs& a, int bx, int by, Src64 (*sh)[kTile],
+                                               const double2* lt = nullptr) {
   double x[R][3];
   int self[R], idx[R];
   load_targets<R>(a, bx, x, self, idx);
@@ -507,13 +565,13 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
         for (int j = 0; j < kTile; j += 4) {
           const Src64 s0 = ts[j], s1 = ts[j + 1], s2 = ts[j + 2], s3 = ts[j + 3];
 #pragma unroll
-          for (int r = 0; r < R; ++r) dot_terms4<PSI>(x[r], s0, s1, s2, s3, S[r]);
+          for (int r = 0; r < R; ++r) dot_terms4<PSI>(x[r], s0, s1, s2, s3, S[r], lt);
         }
       } else {
         for (int j = 0; j < kTile; ++j) {
           const Src64 sj = ts[j];
 #pragma unroll
-          for (int r = 0; r < R; ++r) dot_term<PSI, true>(x[r], sj, base + j == self[r], S[r]);
+          for (int r = 0; r < R; ++r) dot_term<PSI, true>(x[r], sj, base + j == self[r], S[r], lt);
         }
       }
     } else {
@@ -533,14 +591,14 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
         for (int j = 0; j < kTile; ++j) {
           const Src64 sj = ts[j];
 #pragma unroll
-          for (int r = 0; r < R; ++r) near_term64<PSI, true>(D[r], sj, base + j == self[r], S[r]);
+          for (int r = 0; r < R; ++r) near_term64<PSI, true>(D[r], sj, base + j == self[r], S[r], lt);
         }
       } else if (__ballot(near) != 0) {
 #pragma unroll 2
         for (int j = 0; j < kTile; ++j) {
           const Src64 sj = ts[j];
 #pragma unroll
-          for (int r = 0; r < R; ++r) near_term64<PSI, false>(D[r], sj, false, S[r]);
+          for (int r = 0; r < R; ++r) near_term64<PSI, false>(D[r], sj, false, S[r], lt);
         }
       } else {
         double A[R], F[R][kNC];
@@ -552,7 +610,7 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
         for (int j = 0; j < kTile; j += 4) {
           const Src64 s0 = ts[j], s1 = ts[j + 1], s2 = ts[j + 2], s3 = ts[j + 3];
 #pragma unroll
-          for (int r = 0; r < R; ++r) far_terms4_64<PSI>(x[r], A[r], s0, s1, s2, s3, F[r]);
+          for (int r = 0; r < R; ++r) far_terms4_64<PSI>(x[r], A[r], s0, s1, s2, s3, F[r], lt);
         }
         const double4* fr = ctr + a.ntiles + 3 * (long)t;
         const double4 fn = fr[0], f1 = fr[1], f2 = fr[2];
@@ -574,11 +632,18 @@ __device__ __forceinline__ void pair_block_f64(const Src64* __restrict__ src, co
 template <int R, int FORM, bool PSI, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_pair_f64(const Src64* __restrict__ src, const double4* __restrict__ ctr,
                                                      PairArgs a) {
+  // with PSI, the logarithm table of plog (2 KB, filled before the first
+  // barrier of the source loop)
+  __shared__ double2 lt[PSI ? kLogTable : 1];
+  if constexpr (PSI) {
+    log_table_fill(lt);
+    __syncthreads();
+  }
   if constexpr (LDS) {
     __shared__ Src64 sh[2][kTile];
-    pair_block_f64<R, FORM, PSI, true>(src, ctr, a, blockIdx.x, blockIdx.y, sh);
+    pair_block_f64<R, FORM, PSI, true>(src, ctr, a, blockIdx.x, blockIdx.y, sh, lt);
   } else {
-    pair_block_f64<R, FORM, PSI, false>(src, ctr, a, blockIdx.x, blockIdx.y, nullptr);
+    pair_block_f64<R, FORM, PSI, false>(src, ctr, a, blockIdx.x, blockIdx.y, nullptr, lt);
   }
 }
 
@@ -1463,9 +1528,16 @@ int enqueue_prep(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
 // The pair kernel over the request's own grid; ev0/ev1 (optional) bracket it.
 int enqueue_pair(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const PairArgs pa = make_pair_args(q, b);
-  const dim3 grid(cdiv(q.nt, (long)kBlock * q.shape.R), q.shape.splits);
+  // The stream-function kernels hold four more sums and a logarithm per pair:
+  // with fewer targets per lane they keep more waves resident.  Measured at
+  // 1.97M targets (one box): FP64 2,564-2,582 ms at two per lane against
+  // 2,939 at four; FP32C 670 ms at one against 765 at four.  The split
+  // decomposition, and so the partial buffers, stay the shape's.
+  int R = q.shape.R;
+  if (q.want_psi) R = q.prec == LPM_PREC_FP32C ? 1 : std::min(R, 2);
+  const dim3 grid(cdiv(q.nt, (long)kBlock * R), q.shape.splits);
   if (ev0) LPM_HIP(hipEventRecord(ev0, st));
-  switch (q.shape.R) {
+  switch (R) {
     case 1: launch_pair<1>(q, b, pa, grid, st); break;
     case 3: launch_pair<3>(q, b, pa, grid, st); break;
     case 4: launch_pair<4>(q, b, pa, grid, st); break;
