@@ -364,6 +364,12 @@ typedef struct {
 int lpm_bve_write_vtk_arrays(lpm_bve_ctx* ctx, const char* path, int level, double omega, int format,
                              const lpm_vtk_array* arrays, int narrays);
 
+/* lpm_seed_mesh_write_vtk with `narrays` arrays after the leaf areas, host
+ * only: SpherePoisson::outputVtk (src/LpmSpherePoisson.hpp:510-531), which
+ * adds the solution, exact solution and error fields to the mesh. */
+int lpm_seed_mesh_write_vtk_arrays(int seed_kind, int level, const char* seed_path, const char* path, int format,
+                                   const lpm_vtk_array* arrays, int narrays);
+
 /* Total circulation sum over unmasked faces of zeta * area (FP64, device):
  * the conservation check of the Lagrangian scheme; the reference computes it
  * in its drivers from the host mirrors. */

@@ -431,10 +431,34 @@ class SpherePoisson {
     nverts_ = counts[0];
     nfaces_ = counts[2];
     nleaves_ = counts[3];
+    depth_ = depth;
+    seed_kind_ = SeedType::kind;
+    seed_path_ = seed.path;
   }
 
   Index nvertsHost() const { return nverts_; }
   Index nfacesHost() const { return nfaces_; }
+
+  // SpherePoisson::outputVtk (src/LpmSpherePoisson.hpp:510-531): legacy VTK
+  // polydata of the leaf faces (XML for a name ending in .vtp) with the
+  // right-hand side, solution, exact solution and their errors at vertices
+  // and faces, cell "area" first.
+  void outputVtk(const std::string& fname) const {
+    const bool vtp = fname.size() > 4 && fname.compare(fname.size() - 4, 4, ".vtp") == 0;
+    const lpm_vtk_array a[] = {
+        {"f", 1, 0, fverts.data()},          {"psi", 1, 0, psiverts.data()},
+        {"psi_exact", 1, 0, psiexactverts.data()}, {"psi_error", 1, 0, everts.data()},
+        {"velocity", 3, 0, uverts.data()},   {"velocity_exact", 3, 0, uvertsexact.data()},
+        {"velocity_error", 3, 0, uerrverts.data()},
+        {"f", 1, 1, ffaces.data()},          {"psi", 1, 1, psifaces.data()},
+        {"psi_exact", 1, 1, psiexactfaces.data()}, {"psi_error", 1, 1, efaces.data()},
+        {"velocity", 3, 1, ufaces.data()},   {"velocity_exact", 3, 1, ufacesexact.data()},
+        {"velocity_error", 3, 1, uerrfaces.data()}};
+    check(lpm_seed_mesh_write_vtk_arrays(seed_kind_, depth_, seed_path_.empty() ? nullptr : seed_path_.c_str(),
+                                         fname.c_str(), vtp ? LPM_VTK_XML_BINARY : LPM_VTK_LEGACY_ASCII, a,
+                                         (int)(sizeof(a) / sizeof(a[0]))),
+          "outputVtk");
+  }
 
   // average mesh size (src/LpmSpherePoisson.hpp:410-415)
   Real meshSize() const { return std::sqrt(4 * PI / nleaves_); }
@@ -510,6 +534,8 @@ class SpherePoisson {
   Index nmaxv_, nmaxf_;
   int device_;
   Index nverts_ = 0, nfaces_ = 0, nleaves_ = 0;
+  int depth_ = 0, seed_kind_ = LPM_SEED_ICOS_TRI;
+  std::string seed_path_;
 };
 
 }  // namespace Lpm

@@ -52,6 +52,7 @@ SIGNATURES = {
     "lpm_seed_tree_init": [C.c_int, C.c_int, C.c_char_p, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn, _dpn],
     "lpm_seed_mesh_write_nc": [C.c_int, C.c_int, C.c_char_p, C.c_char_p],
     "lpm_seed_mesh_write_vtk": [C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int],
+    "lpm_seed_mesh_write_vtk_arrays": [C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(VtkArray), C.c_int],
     "lpm_ic_gauss_params": [C.c_uint64, C.c_int, C.c_double, _dp, _dp],
     "lpm_ic_eval": [_dp, C.c_int64, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dpn, _dpn,
                     C.c_double],

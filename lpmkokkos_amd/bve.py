@@ -117,6 +117,8 @@ class TreeMesh:
         self.farea = np.zeros(nf)
         self.fmask = np.zeros(nf, np.uint8)
         self.seed_type = seed.seed_type
+        self.seed_kind = seed.kind
+        self.seed_path = seed.path.encode() if seed.path else None
         self.fverts = np.zeros((nf, seed.seed_type.nfaceverts), np.int32)
         self.fparent = np.zeros(nf, np.int32)
         counts = np.zeros(4, np.int64)
@@ -138,6 +140,8 @@ class TreeMesh:
         seed_type = CubedSphereSeed if nv == nl + 2 else IcosTriSphereSeed
         m = cls.__new__(cls)
         m.seed_type = seed_type
+        m.seed_kind = _lib.SEED_CUBED_SPHERE if seed_type is CubedSphereSeed else _lib.SEED_ICOS_TRI
+        m.seed_path = None
         m.vx, m.vlag = np.zeros((nv, 3)), np.zeros((nv, 3))
         m.fx, m.flag = np.zeros((nf, 3)), np.zeros((nf, 3))
         m.farea, m.fmask = np.zeros(nf), np.zeros(nf, np.uint8)
@@ -654,3 +658,23 @@ class SpherePoisson:
                   f"\tpotential: linf_verts = {self.linf_verts:g}, linf_faces = {self.linf_faces:g}\n"
                   f"\tvelocity:  linf_verts = {math.sqrt(self.linf_uverts):g}, "
                   f"linf_faces = {math.sqrt(self.linf_ufaces):g}")
+
+    def outputVtk(self, fname: str):
+        """SpherePoisson::outputVtk (src/LpmSpherePoisson.hpp:510-531): legacy VTK
+        polydata of the leaf faces (XML for a name ending in .vtp) with f, psi,
+        psi_exact, psi_error, velocity, velocity_exact and velocity_error at
+        vertices and faces, cell "area" first."""
+        m = self.mesh
+        fields = [("f", self.fverts, self.ffaces), ("psi", self.psiverts, self.psifaces),
+                  ("psi_exact", self.psiexactverts, self.psiexactfaces), ("psi_error", self.everts, self.efaces),
+                  ("velocity", self.uverts, self.ufaces), ("velocity_exact", self.uvertsexact, self.ufacesexact),
+                  ("velocity_error", self.uerrverts, self.uerrfaces)]
+        rows = [(name, False, v) for name, v, _ in fields] + [(name, True, f) for name, _, f in fields]
+        keep = []
+        items = (_lib.VtkArray * len(rows))()
+        for i, (name, on_faces, values) in enumerate(rows):
+            a = np.ascontiguousarray(values, np.float64)
+            keep.append(a)
+            items[i] = _lib.VtkArray(name.encode(), 1 if a.ndim == 1 else int(a.shape[1]), int(on_faces), a.ctypes.data)
+        fmt = _lib.VTK_XML_BINARY if fname.endswith(".vtp") else _lib.VTK_LEGACY_ASCII
+        call("lpm_seed_mesh_write_vtk_arrays", m.seed_kind, m.level, m.seed_path, fname.encode(), fmt, items, len(rows))

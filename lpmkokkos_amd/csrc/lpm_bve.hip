@@ -2767,22 +2767,15 @@ int lpm_bve_write_vtk_arrays(lpm_bve_ctx* c, const char* path, int level, double
   if (!path) return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: null path");
   if (format < LPM_VTK_LEGACY_ASCII || format > LPM_VTK_XML_BINARY)
     return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: unknown format");
-  if (narrays < 0 || (narrays > 0 && !arrays)) return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: bad array list");
   std::vector<VtkExtra> extra;
-  for (int k = 0; k < narrays; ++k) {
-    const lpm_vtk_array& a = arrays[k];
-    if (!a.name || !a.name[0] || std::strlen(a.name) > 255 || a.ncomp < 1 || a.ncomp > 9 || !a.data)
-      return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: array " + std::to_string(k) +
-                                   ": name must have 1 to 255 characters, 1 <= ncomp <= 9, data non-null");
-    extra.push_back({a.name, a.ncomp, a.on_faces != 0, a.data});
-  }
+  std::string err;
+  if (!vtk_extras(arrays, narrays, extra, err)) return fail(LPM_ERR_ARG, "lpm_bve_write_vtk: " + err);
   LPM_HIP(hipSetDevice(c->device));
   std::unique_ptr<TreeMesh> tree;
   int rc = ctx_tree(c, level, true, tree, "lpm_bve_write_vtk");
   if (rc) return rc;
   BveFields s;
   if ((rc = ctx_download(c, omega, s))) return rc;
-  std::string err;
   if (!write_bve_vtk(path, s, *tree, format, err, extra)) return fail(LPM_ERR_IO, "lpm_bve_write_vtk: " + err);
   return ok();
 }

@@ -146,6 +146,21 @@ int lpm_seed_mesh_write_vtk(int kind, int level, const char* seed_path, const ch
   return ok();
 }
 
+int lpm_seed_mesh_write_vtk_arrays(int kind, int level, const char* seed_path, const char* path, int format,
+                                   const lpm_vtk_array* arrays, int narrays) {
+  if (!path) return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: null path");
+  if (format < LPM_VTK_LEGACY_ASCII || format > LPM_VTK_XML_BINARY)
+    return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: unknown format");
+  std::vector<VtkExtra> extra;
+  std::string err;
+  if (!vtk_extras(arrays, narrays, extra, err)) return fail(LPM_ERR_ARG, "lpm_mesh_write_vtk: " + err);
+  std::unique_ptr<TreeMesh> tm;
+  const int rc = make_tree(kind, level, seed_path, "lpm_mesh_write_vtk", tm);
+  if (rc) return rc;
+  if (!write_mesh_vtk(path, *tm, format, err, extra)) return fail(LPM_ERR_IO, "lpm_mesh_write_vtk: " + err);
+  return ok();
+}
+
 int lpm_mesh_write_vtk(int level, const char* seed_path, const char* path, int format) {
   return lpm_seed_mesh_write_vtk(kSeedIcosTri, level, seed_path, path, format);
 }

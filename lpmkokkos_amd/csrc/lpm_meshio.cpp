@@ -356,6 +356,25 @@ bool write_polydata(const std::string& path, int format, int64_t nv, const doubl
   return true;
 }
 
+// Caller arrays after a writer's own: point arrays as given, face arrays
+// (tree order) restricted to the leaf rows, kept alive in `leaf_rows`.
+void add_extras(const std::vector<VtkExtra>& extra, const std::vector<int64_t>& leaves, std::vector<Array>& pdata,
+                std::vector<Array>& cdata, std::vector<std::vector<double>>& leaf_rows) {
+  const int64_t nc = (int64_t)leaves.size();
+  leaf_rows.reserve(extra.size());
+  for (const VtkExtra& x : extra) {
+    if (!x.faces) {
+      pdata.push_back({x.name.c_str(), x.ncomp, x.data});
+      continue;
+    }
+    leaf_rows.emplace_back((size_t)(x.ncomp * nc));
+    std::vector<double>& rows = leaf_rows.back();
+    for (int64_t c = 0; c < nc; ++c)
+      for (int k = 0; k < x.ncomp; ++k) rows[(size_t)(x.ncomp * c + k)] = x.data[x.ncomp * leaves[(size_t)c] + k];
+    cdata.push_back({x.name.c_str(), x.ncomp, rows.data()});
+  }
+}
+
 }  // namespace
 
 bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& m, int format, std::string& err,
@@ -389,27 +408,37 @@ bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& 
                               {n_psi, 1, c_psi.data()},
                               {"velocity", 3, c_u.data()}};
   std::vector<std::vector<double>> extra_leaf;  // leaf rows of face arrays
-  extra_leaf.reserve(extra.size());
-  for (const VtkExtra& x : extra) {
-    if (!x.faces) {
-      pdata.push_back({x.name.c_str(), x.ncomp, x.data});
-      continue;
-    }
-    extra_leaf.emplace_back((size_t)(x.ncomp * nc));
-    std::vector<double>& rows = extra_leaf.back();
-    for (int64_t c = 0; c < nc; ++c)
-      for (int k = 0; k < x.ncomp; ++k) rows[(size_t)(x.ncomp * c + k)] = x.data[x.ncomp * leaves[(size_t)c] + k];
-    cdata.push_back({x.name.c_str(), x.ncomp, rows.data()});
-  }
+  add_extras(extra, leaves, pdata, cdata, extra_leaf);
   return write_polydata(path, format, s.nv, s.vx.data(), m, leaves, pdata, cdata, "LPM BVE sphere", err);
 }
 
-bool write_mesh_vtk(const std::string& path, const TreeMesh& m, int format, std::string& err) {
+bool vtk_extras(const lpm_vtk_array* arrays, int narrays, std::vector<VtkExtra>& out, std::string& err) {
+  out.clear();
+  if (narrays < 0 || (narrays > 0 && !arrays)) {
+    err = "bad array list";
+    return false;
+  }
+  for (int k = 0; k < narrays; ++k) {
+    const lpm_vtk_array& a = arrays[k];
+    if (!a.name || !a.name[0] || std::strlen(a.name) > 255 || a.ncomp < 1 || a.ncomp > 9 || !a.data) {
+      err = "array " + std::to_string(k) + ": name must have 1 to 255 characters, 1 <= ncomp <= 9, data non-null";
+      return false;
+    }
+    out.push_back({a.name, a.ncomp, a.on_faces != 0, a.data});
+  }
+  return true;
+}
+
+bool write_mesh_vtk(const std::string& path, const TreeMesh& m, int format, std::string& err,
+                    const std::vector<VtkExtra>& extra) {
   const std::vector<int64_t> leaves = leaf_faces(m);
   std::vector<double> area(leaves.size());
   for (size_t c = 0; c < leaves.size(); ++c) area[c] = m.face_area[(size_t)leaves[c]];
-  return write_polydata(path, format, m.nverts(), m.vert_phys.data()->v, m, leaves, {}, {{"area", 1, area.data()}},
-                        "LPM PolyMesh2d", err);
+  std::vector<Array> pdata, cdata = {{"area", 1, area.data()}};
+  std::vector<std::vector<double>> extra_leaf;
+  add_extras(extra, leaves, pdata, cdata, extra_leaf);
+  return write_polydata(path, format, m.nverts(), m.vert_phys.data()->v, m, leaves, pdata, cdata, "LPM PolyMesh2d",
+                        err);
 }
 
 }  // namespace lpm

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "lpm_bve.h"
 #include "lpm_mesh.hpp"
 #include "lpm_ncio.hpp"
 
@@ -74,8 +75,14 @@ struct VtkExtra {
 bool write_bve_vtk(const std::string& path, const BveFields& s, const TreeMesh& mesh, int format,
                    std::string& err, const std::vector<VtkExtra>& extra = {});
 
+// The caller arrays of the C ABI as VtkExtra: false (err set) unless every
+// name has 1 to 255 bytes, 1 <= ncomp <= 9 and data is non-null.
+bool vtk_extras(const lpm_vtk_array* arrays, int narrays, std::vector<VtkExtra>& out, std::string& err);
+
 // PolyMesh2d::outputVtk (src/LpmPolyMesh2d.cpp:79-85): the mesh's leaf faces
-// with their areas as cell data ("area").
-bool write_mesh_vtk(const std::string& path, const TreeMesh& mesh, int format, std::string& err);
+// with their areas as cell data ("area"), then `extra` arrays (the fields
+// SpherePoisson::outputVtk adds, src/LpmSpherePoisson.hpp:510-531).
+bool write_mesh_vtk(const std::string& path, const TreeMesh& mesh, int format, std::string& err,
+                    const std::vector<VtkExtra>& extra = {});
 
 }  // namespace lpm

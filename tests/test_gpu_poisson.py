@@ -91,3 +91,49 @@ def test_python_sphere_poisson_cubed(depth):
     want = TABLE_CUBED[depth]
     assert sig6(sp.meshSize() * 180 / np.pi, want[0])
     assert sig6(sp.linf_verts, want[1]) and sig6(sp.linf_faces, want[2])
+
+
+@pytest.mark.parametrize("seed_type,ext", [("cubed", ".vtk"), ("icos", ".vtp")])
+def test_sphere_poisson_output_vtk(tmp_path, seed_type, ext):
+    """SpherePoisson::outputVtk (src/LpmSpherePoisson.hpp:510-531), as the
+    reference's examples/SphereVorticity.cpp calls it after solve: the leaf
+    mesh with f, psi, psi_exact, psi_error, velocity, velocity_exact and
+    velocity_error at vertices and faces (cell area first), values as solved."""
+    from test_gpu_files import read_legacy, read_xml
+    seed = L.MeshSeed(L.CubedSphereSeed) if seed_type == "cubed" else L.MeshSeed()
+    nv, ne, nf = seed.setMaxAllocations(3)
+    sp = L.SpherePoisson(nv, ne, nf)
+    sp.treeInit(3, seed)
+    sp.init()
+    sp.solve(quiet=True)
+    path = str(tmp_path / ("sphere_poisson_3" + ext))
+    sp.outputVtk(path)
+    d = read_xml(path) if ext == ".vtp" else read_legacy(path)
+    names = ["f", "psi", "psi_exact", "psi_error", "velocity", "velocity_exact", "velocity_error"]
+    assert list(d["point"]) == names and list(d["cell"]) == ["area"] + names
+    lv = sp.mesh.fmask == 0
+    assert np.array_equal(d["point"]["psi"].ravel(), sp.psiverts)
+    assert np.array_equal(d["cell"]["psi_error"].ravel(), sp.efaces[lv])
+    assert np.array_equal(d["cell"]["velocity"].reshape(-1, 3), sp.ufaces[lv])
+    assert np.array_equal(d["cell"]["area"].ravel(), sp.mesh.farea[lv])
+    assert np.abs(d["point"]["psi_error"]).max() == sp.linf_verts
+
+
+def test_poisson_driver_writes_vtk(tmp_path):
+    """poisson_driver with a file root writes each depth's solution as
+    SpherePoisson::outputVtk does (the C++ class, both seeds)."""
+    from test_gpu_files import read_legacy
+    exe = os.path.join(ROOT, "lpmkokkos_amd", "poisson_driver")
+    root = str(tmp_path / "sp_")
+    out = subprocess.run([exe, "2", root], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert names == sorted(f"sp_{k}{d}.vtk" for k in ("icostri", "cubedsphere") for d in range(3))
+    d = read_legacy(str(tmp_path / "sp_cubedsphere2.vtk"))
+    m = L.TreeMesh(2, L.MeshSeed(L.CubedSphereSeed))
+    assert np.array_equal(d["polys"][:, 1:], m.fverts[m.fmask == 0])
+    assert list(d["cell"]) == ["area", "f", "psi", "psi_exact", "psi_error", "velocity", "velocity_exact",
+                               "velocity_error"]
+    # the printed L-infinity error is the file's largest vertex psi error
+    ev = float(re.findall(r"cubedsphere: [0-9.e+-]+:\s*\n\s*potential: linf_verts = ([0-9.e+-]+)", out.stdout)[2])
+    assert sig6(np.abs(d["point"]["psi_error"]).max(), ev)
