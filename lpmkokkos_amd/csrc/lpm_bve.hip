@@ -116,8 +116,9 @@ __global__ __launch_bounds__(kBlock) void k_prep_f64_dot(Src64* __restrict__ src
 // out of q (y - x)/|d|^2 is cancelled by the cross product with x, and the
 // large coefficient (along n) is rounded relative to itself, so its error
 // stays in proportion to the term's own size after the cross product.
-// |d|^2 is formed in FP32 from the stored offsets so that both pair forms
-// see the same source position.
+// b.w = |d|^2 (FP32, from the stored offsets) is not read by the pair
+// kernels, which form the far denominator (1 - x.c) - x.d and the near one
+// from d; it fills the record to 32 bytes.
 struct Src32 {
   float4 a, b;
 };
@@ -665,21 +666,6 @@ __device__ __forceinline__ void pair32_near(const float (&D)[3], const Src32& s,
   if (PSI) T[3] = fmaf(q, __builtin_amdgcn_logf(r2), T[3]);
 }
 
-// Far form with psi: |d|^2 = |D|^2 + |dx|^2 + 2 D.dx from the precomputed
-// |dx|^2; F[0..2] accumulate q y/|d|^2 in frame components (see Src32),
-// F[3] accumulates q log2 |d|^2.
-__device__ __forceinline__ void pair32_far_psi(const float (&D2)[3], float E, const Src32& s, float (&F)[kNC]) {
-  float r2 = E + s.b.w;
-  r2 = fmaf(D2[0], s.a.x, r2);
-  r2 = fmaf(D2[1], s.a.y, r2);
-  r2 = fmaf(D2[2], s.a.z, r2);
-  const float inv = __builtin_amdgcn_rcpf(r2);
-  F[0] = fmaf(inv, s.a.w, F[0]);
-  F[1] = fmaf(inv, s.b.x, F[1]);
-  F[2] = fmaf(inv, s.b.y, F[2]);
-  F[3] = fmaf(s.b.z, __builtin_amdgcn_logf(r2), F[3]);
-}
-
 // S += scale (n F0 + e1 F1 + e2 F2) in FP64 with the tile frame fr[0..2].
 __device__ __forceinline__ void add_frame(double (&S)[kNC], const double4* fr, float F0, float F1, float F2,
                                           double scale) {
@@ -797,26 +783,45 @@ __device__ __forceinline__ void far_tile_sum(const Src32* ts, const double (&x)[
   for (int r = 0; r < R; ++r) add_frame(S[r], fr, F[r][0], F[r][1], F[r][2], 0.5);
 }
 
-// A tile's far-form contribution with psi for R targets, added to S.
+// A tile's far-form contribution with psi for R targets, added to S: the
+// velocity kernel's denominators den = (1 - x.c) - x.d = |d|^2/2 and
+// reciprocals (one per four sources of a target; far denominators are
+// bounded below, padding ones are 1), and P = sum q log2(den).  The
+// stream-function sums use q log2 |d|^2 = q log2(den) + q, so the tile adds
+// P plus its FP64 sum of q (ctr[t].w).
 template <int R>
-__device__ __forceinline__ void far_tile_psi(const Src32* ts, const float (&D)[R][3], const float (&E)[R],
-                                             const double4* fr, double (&S)[R][kNC]) {
-  float D2[R][3], F[R][kNC];
+__device__ __forceinline__ void far_tile_psi(const Src32* ts, const double (&x)[R][3], const double4& c,
+                                             const double4* fr, const float (&NX)[R][3], double (&S)[R][kNC]) {
+  float A[R], F[R][3], P[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    for (int k = 0; k < 3; ++k) D2[r][k] = 2.0f * D[r][k];
-    for (int k = 0; k < kNC; ++k) F[r][k] = 0.0f;
+    A[r] = (float)(1.0 - (x[r][0] * c.x + x[r][1] * c.y + x[r][2] * c.z));
+    for (int k = 0; k < 3; ++k) F[r][k] = 0.0f;
+    P[r] = 0.0f;
   }
-#pragma unroll 4
-  for (int j = 0; j < kTile; ++j) {
-    const Src32 sj = ts[j];
+  for (int j = 0; j < kTile; j += 4) {
+    const Src32 s0 = ts[j], s1 = ts[j + 1], s2 = ts[j + 2], s3 = ts[j + 3];
 #pragma unroll
-    for (int r = 0; r < R; ++r) pair32_far_psi(D2[r], E[r], sj, F[r]);
+    for (int r = 0; r < R; ++r) {
+      const float a = far_den(NX[r], A[r], s0), b = far_den(NX[r], A[r], s1);
+      const float cc = far_den(NX[r], A[r], s2), d = far_den(NX[r], A[r], s3);
+      const float ab = a * b, cd = cc * d;
+      const float p = __builtin_amdgcn_rcpf(ab * cd);
+      const float iab = cd * p, icd = ab * p;
+      far_acc(b * iab, s0, F[r]);
+      far_acc(a * iab, s1, F[r]);
+      far_acc(d * icd, s2, F[r]);
+      far_acc(cc * icd, s3, F[r]);
+      P[r] = fmaf(s0.b.z, __builtin_amdgcn_logf(a), P[r]);
+      P[r] = fmaf(s1.b.z, __builtin_amdgcn_logf(b), P[r]);
+      P[r] = fmaf(s2.b.z, __builtin_amdgcn_logf(cc), P[r]);
+      P[r] = fmaf(s3.b.z, __builtin_amdgcn_logf(d), P[r]);
+    }
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    add_frame(S[r], fr, F[r][0], F[r][1], F[r][2], 1.0);
-    S[r][3] += (double)F[r][3];
+    add_frame(S[r], fr, F[r][0], F[r][1], F[r][2], 0.5);
+    S[r][3] += (double)P[r] + c.w;
   }
 }
 
@@ -883,7 +888,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c(const Src32* __restrict__ 
       for (int r = 0; r < R; ++r)
         for (int k = 0; k < (PSI ? kNC : 3); ++k) S[r][k] += (double)T[r][k];
     } else if constexpr (PSI) {
-      far_tile_psi<R>(ts, D, E, fr, S);
+      far_tile_psi<R>(ts, x, c, fr, NX, S);
     } else {
       far_tile_sum<R>(ts, x, c, fr, NX, S);
     }
@@ -970,7 +975,7 @@ __device__ __forceinline__ void pair_block_f32c_lds(const Src32* __restrict__ sr
       for (int r = 0; r < R; ++r)
         for (int k = 0; k < (PSI ? kNC : 3); ++k) S[r][k] += (double)T[r][k];
     } else if constexpr (PSI) {
-      far_tile_psi<R>(ts, D, E, fr, S);
+      far_tile_psi<R>(ts, x, c, fr, NX, S);
     } else {
       far_tile_sum<R>(ts, x, c, fr, NX, S);
     }
@@ -1529,12 +1534,13 @@ int enqueue_prep(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
 int enqueue_pair(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const PairArgs pa = make_pair_args(q, b);
   // The stream-function kernels hold four more sums and a logarithm per pair:
-  // with fewer targets per lane they keep more waves resident.  Measured at
-  // 1.97M targets (one box): FP64 2,564-2,582 ms at two per lane against
-  // 2,939 at four; FP32C 670 ms at one against 765 at four.  The split
-  // decomposition, and so the partial buffers, stay the shape's.
+  // with at most two targets per lane they keep more waves resident.
+  // Measured at 1.97M targets (one box): FP64 2,564-2,582 ms at two per lane
+  // against 2,939 at four; FP32C 626 ms at two against 707 at four and 819
+  // at one.  The split decomposition, and so the partial buffers, stay the
+  // shape's.
   int R = q.shape.R;
-  if (q.want_psi) R = q.prec == LPM_PREC_FP32C ? 1 : std::min(R, 2);
+  if (q.want_psi) R = std::min(R, 2);
   const dim3 grid(cdiv(q.nt, (long)kBlock * R), q.shape.splits);
   if (ev0) LPM_HIP(hipEventRecord(ev0, st));
   switch (R) {
