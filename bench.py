@@ -18,6 +18,9 @@ Also reported, in the same line:
     kernel and stream function at the final FP64 state), the FP64 reference
     scheme's rate, and the pair kernel's roofline position (HIP events around
     the kernel on its own stream);
+  * the reference-side copy-in/copy-out binding of advance_timestep on host
+    arrays (one FP64 reference-scheme step per call): its PCIe-inclusive rate
+    and the time its copies add to the resident step;
   * one timed FP64 reference RK4 step at level 7 (the same field);
   * the batching configuration in full (64 members, levels 3-7, 10 FP32C
     steps): ensemble rate, the members' sequential rate, and whether every
@@ -65,6 +68,38 @@ OMEGA = 2 * np.pi
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def drop_in_binding(ctx, farea, nl: int, pairs_step: int, resident_s: float, calls: int = 2):
+    """The reference-side binding of advance_timestep on host arrays: copy the
+    state and entry velocity in (lpm_bve_set_state + lpm_bve_set_velocity, in
+    place after the first call), one FP64 reference-scheme step, copy the state
+    out (lpm_bve_get_state) -- the PCIe-inclusive rate of the drop-in path.
+    One untimed call, then `calls` timed ones."""
+    from lpmkokkos_amd import _lib
+    import lpmkokkos_amd as L
+    s = ctx.get_state()
+    mask = np.zeros(nl, np.uint8)
+    ts = []
+    for i in range(calls + 1):
+        t = time.perf_counter()
+        ctx.set_state(s["vx"], s["vzeta"], s["fx"], s["fzeta"], farea, mask)
+        ctx.set_velocity(s["vu"], s["fu"])
+        ctx.advance(DT, OMEGA, L.SCHEME_REF, L.PREC_FP64, nsteps=1)
+        _lib.call("lpm_bve_get_state", ctx.h, _lib.ptr(s["vx"]), _lib.ptr(s["vzeta"]), _lib.ptr(s["vu"]), None,
+                  _lib.ptr(s["fx"]), _lib.ptr(s["fzeta"]), _lib.ptr(s["fu"]), None)
+        if i:
+            ts.append(time.perf_counter() - t)
+    t = min(ts)
+    return {
+        "workload": "one FP64 reference-scheme step per call on host arrays: set_state + set_velocity, "
+                    "advance, get_state (state in place on the device after the first call)",
+        "ms_per_step": 1e3 * t,
+        "resident_ms_per_step": 1e3 * resident_s,
+        "copy_ms_per_step": 1e3 * (t - resident_s),
+        "pairs_per_s": pairs_step / t,
+        "calls_timed": calls,
+    }
 
 
 def build_problem(level: int, seed: int = 1):
@@ -125,7 +160,7 @@ def ensemble_run(nmembers: int = 64, nsteps: int = 10) -> dict:
     """Batching configuration: 64 members, levels uniform on 3..7 (seed 3), RH
     wave (K x U(0.95, 1.05)) + 8 Gaussian vortices, 10 FP32C steps, one
     persistent pair kernel per evaluation, step graph replayed, launch shapes
-    from the autotuner.  Every member is then advanced alone by the same 10
+    and the persistent grid (workgroups per CU) from the autotuners.  Every member is then advanced alone by the same 10
     steps (one at a time, as a caller without batching would run them: the
     sequential rate) and compared with its ensemble result bit for bit."""
     import lpmkokkos_amd as L
@@ -142,6 +177,10 @@ def ensemble_run(nmembers: int = 64, nsteps: int = 10) -> dict:
     ens = E.Ensemble(0)
     for c in members:
         ens.add(c)
+    # waves per CU of the persistent kernel: the grid autotuner's rule for this total N
+    grid = ens.autotune(L.PREC_FP32C, L.SCHEME_LAGR, reps=1)
+    log(f"[bench] ensemble: persistent grid autotuned: {grid[0]} workgroups per CU "
+        f"(ms per evaluation by workgroups per CU: {', '.join(f'{k}: {v:.1f}' for k, v in grid[2].items())})")
     ens.advance(DT, OMEGA, nsteps=1)  # warm: entry velocities, plan, capture
     ens.synchronize()
     t1 = time.perf_counter()
@@ -182,6 +221,8 @@ def ensemble_run(nmembers: int = 64, nsteps: int = 10) -> dict:
         "members_bitwise_equal_alone": f"{same}/{len(specs)}",
         "all_members_bitwise_equal_alone": same == len(specs),
         "autotuned_shapes": {int(k): list(v) for k, v in tuned.items()},
+        "autotuned_workgroups_per_cu": grid[0],
+        "eval_ms_by_workgroups_per_cu": grid[2],
     }
 
 
@@ -306,7 +347,7 @@ def main() -> int:
     # FP64 and FP32C from the same initial state over --compare-steps steps (the FP64 run is
     # also the FP64 rate), then FP32C error against FP64: positions after the steps, velocity
     # from the step kernel and stream function at the final FP64 state
-    rate64 = rate64_ref = err = None
+    rate64 = rate64_ref = err = binding = None
     nsteps_cmp = args.compare_steps
     if nsteps_cmp > 0:
         c64 = L.Context(0)
@@ -361,6 +402,9 @@ def main() -> int:
         t_ref = time.perf_counter() - t3
         rate64_ref = pairs_step / t_ref
         log(f"[bench] FP64 reference scheme: {1e3 * t_ref:.1f} ms/step, {rate64_ref:.4e} pairs/s")
+        binding = drop_in_binding(c64, farea, nl, pairs_step, t_ref)
+        log(f"[bench] copy-in/copy-out binding: {binding['ms_per_step']:.1f} ms/step "
+            f"({binding['copy_ms_per_step']:.1f} ms of copies), {binding['pairs_per_s']:.4e} pairs/s")
         c64.close()
     ctx.close()
 
@@ -406,6 +450,7 @@ def main() -> int:
         "rk4_steps_per_s": 1e3 / ms_step,
         "fp64_pairs_per_s": rate64,
         "fp64_reference_scheme_pairs_per_s": rate64_ref,
+        "drop_in_binding_pcie_inclusive": binding,
         "fp32c_vs_fp64_speedup": rate / rate64 if rate64 else None,
         "fp32c_vs_fp64_speedup_bound": FP32C_VS_FP64_ISSUE_BOUND,
         "fp32c_vs_fp64_speedup_bound_note": "instruction-issue limit of the two pair loops: FP64 FMA issues at half "

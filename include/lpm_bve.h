@@ -452,8 +452,23 @@ int lpm_ens_destroy(lpm_bve_ensemble* ens);
 int lpm_ens_add(lpm_bve_ensemble* ens, lpm_bve_ctx* member);
 int lpm_ens_remove(lpm_bve_ensemble* ens, lpm_bve_ctx* member);
 int lpm_ens_size(lpm_bve_ensemble* ens, int* nmembers);
-/* Workgroups per CU of the persistent kernel (0 = occupancy limit). */
+/* Workgroups per CU of the persistent kernel (0 = automatic: the
+ * lpm_ens_autotune rule for the ensemble's total target count N, else the
+ * occupancy limit). */
 int lpm_ens_set_blocks_per_cu(lpm_bve_ensemble* ens, int blocks_per_cu);
+/* No counterpart in the reference (the batching form's autotuner).
+ * Time one evaluation of every member at its current state per candidate
+ * persistent grid -- 1 up to the occupancy limit of workgroups (256 lanes,
+ * 4 waves each) per CU -- and record the fastest as the process-wide rule
+ * for ensembles of this total target count N (the members' targets summed).
+ * Results do not depend on the grid.  ms_by_blocks[k] (k < nms; NULL when
+ * nms is 0) receives the mean time at k + 1 workgroups per CU, -1 where not
+ * tried.  The states are left alone; each member's next step recomputes its
+ * entry velocity.  Outputs may be NULL. */
+int lpm_ens_autotune(lpm_bve_ensemble* ens, int prec, int scheme, int reps, int* best_blocks_per_cu,
+                     double* best_ms, double* ms_by_blocks, int nms);
+/* Forget every ensemble grid rule. */
+int lpm_ens_rules_clear(void);
 /* `nsteps` RK4 steps of every member (same dt, omega, scheme, precision). */
 int lpm_ens_advance(lpm_bve_ensemble* ens, double dt, double omega, int scheme, int prec, int nsteps,
                     int use_graph);

@@ -1392,6 +1392,9 @@ struct ShapeRule {
 };
 std::mutex g_rule_mu;
 std::map<int, ShapeRule> g_rules;
+// Persistent-grid rules of ensembles: total target count N of the members ->
+// workgroups (of kBlock lanes, 4 waves) per CU, from lpm_ens_autotune.
+std::map<int64_t, int> g_ens_rules;
 
 Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
   if (R_req == 0) {
@@ -2891,6 +2894,7 @@ struct lpm_bve_ensemble {
   // a new autotuned rule) rebuilds the plan and its graphs
   std::vector<std::tuple<int, int, int, float>> plan_shapes;
   int plan_prec = -1, plan_form = -1;
+  int plan_per_cu = 0;                     // workgroups per CU the plan's grids were sized for
   std::vector<int> plan_R;                 // distinct R values, one launch each
   std::vector<int> plan_nitems;            // items per R launch
   std::vector<int> plan_grid;              // persistent workgroups per R launch
@@ -2939,8 +2943,11 @@ std::tuple<int, int, int, float> shape_key(const lpm_bve_ctx* c) {
   return std::make_tuple(sh.R, sh.splits, sh.tiles_per_split, sh.near_factor);
 }
 
+int ens_blocks_per_cu(const lpm_bve_ensemble* e);
+
 bool ens_plan_current(const lpm_bve_ensemble* e, int prec, int form) {
   if (!e->plan_valid || e->plan_prec != prec || e->plan_form != form) return false;
+  if (e->plan_per_cu != ens_blocks_per_cu(e)) return false;
   if (e->plan_generations.size() != e->members.size() || e->plan_shapes.size() != e->members.size()) return false;
   for (size_t m = 0; m < e->members.size(); ++m)
     if (e->plan_generations[m] != e->members[m]->generation || e->plan_shapes[m] != shape_key(e->members[m]))
@@ -2955,6 +2962,7 @@ int ens_grid(lpm_bve_ensemble* e, int R, int prec, int form, int nitems);
 // cost so that the persistent workgroups take the long items first.
 int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
   ens_drop_plan(e);
+  e->plan_per_cu = ens_blocks_per_cu(e);
   const size_t M = e->members.size();
   std::vector<EnsMember> hw(M), hx(M);
   std::map<int, std::vector<std::pair<long, EnsItem>>> byR;
@@ -3067,17 +3075,37 @@ const void* ens_kernel(int prec, int form) {
   return (const void*)&k_pair_f64_ens<R, FORM_CHORD>;
 }
 
+int64_t ens_total_targets(const lpm_bve_ensemble* e) {
+  int64_t n = 0;
+  for (const auto* c : e->members) n += c->nt;
+  return n;
+}
+
+// Workgroups per CU an ensemble runs its persistent kernels with: its own
+// setting, else the autotuned rule for its total N, else 0 (occupancy).
+int ens_blocks_per_cu(const lpm_bve_ensemble* e) {
+  if (e->blocks_per_cu > 0) return e->blocks_per_cu;
+  std::lock_guard<std::mutex> g(g_rule_mu);
+  auto it = g_ens_rules.find(ens_total_targets(e));
+  return it == g_ens_rules.end() ? 0 : it->second;
+}
+
+// Occupancy limit (workgroups per CU) of the persistent kernel for R.
+int ens_occupancy(int R, int prec, int form) {
+  const void* k = R == 1   ? ens_kernel<1>(prec, form)
+                  : R == 3 ? ens_kernel<3>(prec, form)
+                  : R == 4 ? ens_kernel<4>(prec, form)
+                           : ens_kernel<2>(prec, form);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+  return per_cu;
+}
+
 int ens_grid(lpm_bve_ensemble* e, int R, int prec, int form, int nitems) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, e->device) != hipSuccess) return 256;
-  int per_cu = e->blocks_per_cu;
-  if (per_cu <= 0) {
-    const void* k = R == 1   ? ens_kernel<1>(prec, form)
-                    : R == 3 ? ens_kernel<3>(prec, form)
-                    : R == 4 ? ens_kernel<4>(prec, form)
-                             : ens_kernel<2>(prec, form);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
-  }
+  int per_cu = ens_blocks_per_cu(e);
+  if (per_cu <= 0) per_cu = ens_occupancy(R, prec, form);
   return std::max(1, std::min(nitems, per_cu * prop.multiProcessorCount));
 }
 
@@ -3198,6 +3226,87 @@ int lpm_ens_set_blocks_per_cu(lpm_bve_ensemble* e, int n) {
   LPM_HIP(hipStreamSynchronize(e->stream));
   e->blocks_per_cu = n;
   ens_drop_plan(e);
+  return ok();
+}
+
+// Time one evaluation of every member at its state per candidate persistent
+// grid (1 .. the occupancy limit workgroups per CU) and record the fastest as
+// the rule for ensembles of this total target count.
+int lpm_ens_autotune(lpm_bve_ensemble* e, int prec, int scheme, int reps, int* best_blocks_per_cu, double* best_ms,
+                     double* ms_by_blocks, int nms) {
+  if (!e) return fail(LPM_ERR_ARG, "lpm_ens_autotune: null ensemble");
+  if (scheme != LPM_SCHEME_REF && scheme != LPM_SCHEME_LAGR) return fail(LPM_ERR_ARG, "lpm_ens_autotune: bad scheme");
+  if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32C) return fail(LPM_ERR_ARG, "lpm_ens_autotune: bad precision");
+  if (prec == LPM_PREC_FP32C && scheme != LPM_SCHEME_LAGR)
+    return fail(LPM_ERR_ARG, "lpm_ens_autotune: FP32C pair sums require the renormalised (LAGR) scheme");
+  if (nms < 0 || (nms > 0 && !ms_by_blocks)) return fail(LPM_ERR_ARG, "lpm_ens_autotune: bad time array");
+  if (e->members.empty()) return fail(LPM_ERR_STATE, "lpm_ens_autotune: no members");
+  if (reps < 1) reps = 1;
+  LPM_HIP(hipSetDevice(e->device));
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  for (int k = 0; k < nms; ++k) ms_by_blocks[k] = -1.0;
+  // order after each member's earlier work, as lpm_ens_advance does
+  for (auto* c : e->members) {
+    if (c->timing) return fail(LPM_ERR_STATE, "lpm_ens_autotune: member has pair timing enabled");
+    if (c->stream == e->stream || mark_reached(c->done_flag, c->done_seq)) continue;
+    LPM_HIP(hipEventRecord(e->ev, c->stream));
+    LPM_HIP(hipStreamWaitEvent(e->stream, e->ev, 0));
+  }
+  int rc, occ = 1;
+  for (auto* c : e->members) {
+    if ((rc = ensure_eval_buffers(c, ctx_shape(c)))) return rc;
+    occ = std::max(occ, ens_occupancy(ctx_shape(c).R, prec, form));
+  }
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) {
+    if (t0) (void)hipEventDestroy(t0);
+    return fail(LPM_ERR_HIP, "lpm_ens_autotune: event creation failed");
+  }
+  const int saved = e->blocks_per_cu;
+  double best = 1e300;
+  int best_n = occ;
+  rc = LPM_OK;
+  for (int n = 1; n <= occ && rc == LPM_OK; ++n) {
+    e->blocks_per_cu = n;
+    if ((rc = ens_build_plan(e, prec, form))) break;
+    if ((rc = ens_eval(e, 1, prec, form, 0))) break;  // warm
+    if (hipEventRecord(t0, e->stream) != hipSuccess) rc = fail(LPM_ERR_HIP, "lpm_ens_autotune: event record");
+    for (int r = 0; r < reps && rc == LPM_OK; ++r) rc = ens_eval(e, 1, prec, form, 0);
+    if (rc) break;
+    float ms = 0.0f;
+    if (hipEventRecord(t1, e->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+        hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+      rc = fail(LPM_ERR_HIP, "lpm_ens_autotune: timing failed");
+      break;
+    }
+    const double mean = ms / reps;
+    if (n <= nms) ms_by_blocks[n - 1] = mean;
+    if (mean < best) {
+      best = mean;
+      best_n = n;
+    }
+  }
+  (void)hipStreamSynchronize(e->stream);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  e->blocks_per_cu = saved;
+  ens_drop_plan(e);
+  // the trial evaluations overwrote each member's velocity; its next step
+  // recomputes the entry velocity
+  for (auto* c : e->members) c->vel_valid = false;
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> g(g_rule_mu);
+    g_ens_rules[ens_total_targets(e)] = best_n;
+  }
+  if (best_blocks_per_cu) *best_blocks_per_cu = best_n;
+  if (best_ms) *best_ms = best;
+  return ok();
+}
+
+int lpm_ens_rules_clear(void) {
+  std::lock_guard<std::mutex> g(g_rule_mu);
+  g_ens_rules.clear();
   return ok();
 }
 

@@ -59,7 +59,20 @@ class Ensemble:
         return n.value
 
     def set_blocks_per_cu(self, n: int):
+        """Workgroups per CU of the persistent kernel (0: the autotuned rule for
+        the ensemble's total N, else the occupancy limit)."""
         call("lpm_ens_set_blocks_per_cu", self.h, n)
+
+    def autotune(self, prec: int = PREC_FP32C, scheme: int = SCHEME_LAGR, reps: int = 2, max_blocks: int = 16):
+        """Time one evaluation of every member per persistent grid (1 .. the
+        occupancy limit of workgroups per CU); the fastest becomes the rule for
+        ensembles of this total N (process-wide).  Returns (workgroups per CU,
+        ms, {workgroups per CU: ms} of the candidates tried)."""
+        best, ms = C.c_int(), C.c_double()
+        times = np.zeros(max_blocks)
+        call("lpm_ens_autotune", self.h, prec, scheme, reps, C.byref(best), C.byref(ms), _lib.ptr(times),
+             max_blocks)
+        return best.value, ms.value, {k + 1: float(t) for k, t in enumerate(times) if t >= 0}
 
     def advance(self, dt: float, omega: float, scheme: int = SCHEME_LAGR, prec: int = PREC_FP32C, nsteps: int = 1,
                 use_graph: bool = True):

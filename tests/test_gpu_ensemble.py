@@ -144,6 +144,51 @@ def test_shape_change_rebuilds_plan():
             c.close()
 
 
+def test_grid_autotune_changes_no_bits():
+    """The persistent grid's autotuner (workgroups per CU, the rule keyed on
+    the ensemble's total N) times every candidate, and the ensemble then runs
+    with the chosen grid, with an explicit one and after the rules are
+    cleared; each member still equals its run alone bit for bit, FP32C and
+    FP64."""
+    from lpmkokkos_amd import _lib
+    for prec in (L.PREC_FP32C, L.PREC_FP64):
+        sp = specs()
+        alone = [E.make_member(s) for s in sp]
+        mixed = [E.make_member(s) for s in sp]
+        ens = E.Ensemble(0)
+        for c in mixed:
+            ens.add(c)
+        ens.advance(0.005, TWO_PI, L.SCHEME_LAGR, prec, nsteps=1)
+        best, ms, times = ens.autotune(prec, L.SCHEME_LAGR, reps=1)
+        assert sorted(times) == list(range(1, len(times) + 1)) and best in times
+        assert ms == min(times.values()) and all(t > 0 for t in times.values())
+        ens.advance(0.005, TWO_PI, L.SCHEME_LAGR, prec, nsteps=2)  # plan rebuilt for the rule
+        ens.set_blocks_per_cu(1)
+        ens.advance(0.005, TWO_PI, L.SCHEME_LAGR, prec, nsteps=1)
+        ens.set_blocks_per_cu(0)
+        _lib.call("lpm_ens_rules_clear")
+        ens.advance(0.005, TWO_PI, L.SCHEME_LAGR, prec, nsteps=1)
+        for c in alone:
+            c.advance(0.005, TWO_PI, L.SCHEME_LAGR, prec, nsteps=5)
+        for i, (a, b) in enumerate(zip(alone, mixed)):
+            assert same(a, b), (prec, i)
+        ens.close()
+        for c in alone + mixed:
+            c.close()
+
+
+def test_grid_autotune_errors():
+    ens = E.Ensemble(0)
+    with pytest.raises(RuntimeError):
+        ens.autotune()  # no members
+    c = E.make_member(specs()[0])
+    ens.add(c)
+    with pytest.raises(RuntimeError):
+        ens.autotune(L.PREC_FP32C, L.SCHEME_REF)  # FP32C needs the Lagrangian scheme
+    ens.close()
+    c.close()
+
+
 @pytest.mark.timeout(400)
 def test_batching_configuration_all_members(omesh):
     """The batching configuration in full: 64 members with levels drawn
