@@ -283,7 +283,7 @@ def main() -> int:
     ap.add_argument("--targets-per-lane", type=int, default=0)
     ap.add_argument("--splits", type=int, default=0)
     ap.add_argument("--near-factor", type=float, default=None)
-    ap.add_argument("--staging", choices=["lds", "lds-grid", "scalar"], default=None)
+    ap.add_argument("--staging", choices=["lds", "scalar"], default=None)
     ap.add_argument("--form", choices=["precision", "ensemble", "serving"], default="precision")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-max-level", type=int, default=8)
@@ -318,8 +318,7 @@ def main() -> int:
     if args.near_factor is not None:
         ctx.set_near_factor(args.near_factor)
     if args.staging:
-        ctx.set_source_staging({"lds": L.STAGING_LDS, "lds-grid": L.STAGING_LDS_GRID,
-                                "scalar": L.STAGING_SCALAR}[args.staging])
+        ctx.set_source_staging(args.staging == "lds")
 
     # FP32C: warmup, then K timed steps replayed from the captured graph
     ctx.solve(L.PREC_FP32C, L.SCHEME_LAGR)

@@ -49,15 +49,6 @@ typedef enum {
   LPM_SCHEME_LAGR = 1
 } lpm_scheme;
 
-/* Source staging of a context's pair sums (lpm_bve_set_source_staging). */
-typedef enum {
-  LPM_STAGING_SCALAR = 0,   /* records read with uniform loads through the scalar cache */
-  LPM_STAGING_LDS = 1,      /* tiles staged through LDS; the FP32C velocity sums run as one
-                               persistent grid that takes the (target block, source split)
-                               items from a counter (default) */
-  LPM_STAGING_LDS_GRID = 2  /* tiles staged through LDS, one workgroup per item */
-} lpm_staging;
-
 /* VTK polydata layouts of lpm_bve_write_vtk. */
 typedef enum {
   LPM_VTK_LEGACY_ASCII = 0,  /* BVESphere::outputVtk (src/LpmBVESphere.cpp:90-113): legacy .vtk */
@@ -407,12 +398,9 @@ int lpm_bve_set_launch(lpm_bve_ctx* ctx, int targets_per_lane, int source_splits
  * holding a target's own source always uses the explicit form. */
 int lpm_bve_set_near_factor(lpm_bve_ctx* ctx, double factor);
 
-/* Stage source tiles through LDS (LPM_STAGING_LDS, default, or
- * LPM_STAGING_LDS_GRID) or read them through the scalar cache
- * (LPM_STAGING_SCALAR); both precisions.  The two LDS modes give the same
- * bits (same decomposition and summation order); they differ only in how
- * workgroups are scheduled. */
-int lpm_bve_set_source_staging(lpm_bve_ctx* ctx, int staging);
+/* Stage source tiles through LDS (1, default) or read them through the
+ * scalar cache (0); both precisions. */
+int lpm_bve_set_source_staging(lpm_bve_ctx* ctx, int lds);
 
 /* The context's HIP stream (hipStream_t) and device id. */
 int lpm_bve_stream(lpm_bve_ctx* ctx, void** stream, int* device);

@@ -19,7 +19,6 @@ LPM_OK = 0
 PREC_FP64, PREC_FP32C = 0, 1
 SCHEME_REF, SCHEME_LAGR = 0, 1
 SEED_ICOS_TRI, SEED_CUBED_SPHERE = 0, 1
-STAGING_SCALAR, STAGING_LDS, STAGING_LDS_GRID = 0, 1, 2
 VTK_LEGACY_ASCII, VTK_LEGACY_BINARY, VTK_XML_ASCII, VTK_XML_BINARY = 0, 1, 2, 3
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

@@ -354,11 +354,8 @@ class Context:
     def set_launch(self, targets_per_lane: int, source_splits: int = 0):
         call("lpm_bve_set_launch", self.h, targets_per_lane, source_splits)
 
-    def set_source_staging(self, staging):
-        """STAGING_LDS (or True; default: LDS tiles, the FP32C velocity sums as
-        one persistent grid), STAGING_LDS_GRID (LDS tiles, one workgroup per
-        item; the same bits) or STAGING_SCALAR (or False: scalar-cache reads)."""
-        call("lpm_bve_set_source_staging", self.h, int(staging))
+    def set_source_staging(self, lds: bool):
+        call("lpm_bve_set_source_staging", self.h, int(lds))
 
     def set_near_factor(self, factor: float):
         call("lpm_bve_set_near_factor", self.h, factor)

@@ -3,7 +3,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1044,36 +1043,6 @@ __device__ __forceinline__ int next_item(int* counter, int* slot) {
   return it;
 }
 
-// Persistent form of the FP32C velocity kernel: a grid of about the
-// occupancy limit of workgroups takes the (target block, source split) items
-// of k_pair_f32c_lds's grid, in that grid's order, from an atomic counter, so
-// the work stays balanced over every CU and XCD to the end of the launch (a
-// plain grid gives each XCD a fixed eighth of the workgroups).  Same
-// decomposition and partial slots: results are bit-identical to
-// k_pair_f32c_lds.  The item index passes through the second tile buffer, so
-// the LDS footprint is k_pair_f32c_lds's (four workgroups per CU at R = 4).
-template <int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair_f32c_lds_pers(
-    const Src32* __restrict__ src, const double4* __restrict__ ctr, const float* __restrict__ trad, PairArgs a,
-    int nbx, int nitems, int* counter) {
-  __shared__ Src32 sh[2][kTile];
-  int* slot = reinterpret_cast<int*>(&sh[1][0]);
-  if constexpr (R >= 3) {
-    __shared__ double xs[3 * R * kBlock];
-    for (;;) {
-      const int it = next_item(counter, slot);
-      if (it >= nitems) break;
-      pair_block_f32c_lds<R, false>(src, ctr, trad, a, it % nbx, it / nbx, sh, xs);
-    }
-  } else {
-    for (;;) {
-      const int it = next_item(counter, slot);
-      if (it >= nitems) break;
-      pair_block_f32c_lds<R, false>(src, ctr, trad, a, it % nbx, it / nbx, sh, nullptr);
-    }
-  }
-}
-
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_pair_f32c_ens(const EnsMember* __restrict__ mem,
                                                           const EnsItem* __restrict__ items, int nitems,
@@ -1408,9 +1377,6 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 struct Shape {
   float near_factor = kNearDefault;
   bool lds = true;  // stage source tiles through LDS (else scalar-cache reads)
-  // LDS staging, FP32C velocity sums of a context: one persistent grid takes
-  // the workgroups' items from a counter (else one workgroup per item)
-  bool persistent = true;
   int R = 2;
   int splits = 1;
   int ntiles = 0;
@@ -1475,7 +1441,6 @@ struct EvalBuffers {
   double* qtot = nullptr;
   double* part = nullptr;
   size_t part_elems = 0;
-  int* counter = nullptr;  // work counter of the persistent pair kernel (contexts only)
 };
 
 // One source sweep over targets `tgt` from sources smap[ns] of the
@@ -1568,43 +1533,6 @@ int enqueue_prep(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
   return LPM_OK;
 }
 
-// CUs of the current device and the occupancy limit of the persistent FP32C
-// velocity kernel for R, looked up once.
-int device_cus() {
-  static std::mutex mu;
-  static std::map<int, int> cus;
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 256;
-  std::lock_guard<std::mutex> g(mu);
-  auto it = cus.find(dev);
-  if (it != cus.end()) return it->second;
-  int n = 0;
-  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-  cus[dev] = n;
-  return n;
-}
-
-template <int R>
-int pers_occupancy() {
-  static std::atomic<int> occ{0};
-  int o = occ.load();
-  if (o > 0) return o;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)&k_pair_f32c_lds_pers<R>, kBlock, 0) !=
-          hipSuccess || o <= 0)
-    o = 1;
-  occ.store(o);
-  return o;
-}
-
-template <int R>
-void launch_pair_pers(const EvalBuffers& b, const PairArgs& pa, dim3 grid, hipStream_t st) {
-  const int nitems = (int)(grid.x * grid.y);
-  const int pgrid = std::max(1, std::min(nitems, pers_occupancy<R>() * device_cus()));
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b.counter);
-  hipLaunchKernelGGL((k_pair_f32c_lds_pers<R>), dim3(pgrid), dim3(kBlock), 0, st, b.src32, b.ctr, b.trad, pa,
-                     (int)grid.x, nitems, b.counter);
-}
-
 // The pair kernel over the request's own grid; ev0/ev1 (optional) bracket it.
 int enqueue_pair(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   const PairArgs pa = make_pair_args(q, b);
@@ -1618,17 +1546,6 @@ int enqueue_pair(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
   if (q.want_psi) R = std::min(R, 2);
   const dim3 grid(cdiv(q.nt, (long)kBlock * R), q.shape.splits);
   if (ev0) LPM_HIP(hipEventRecord(ev0, st));
-  if (q.prec == LPM_PREC_FP32C && !q.want_psi && q.shape.lds && q.shape.persistent && b.counter) {
-    switch (R) {
-      case 1: launch_pair_pers<1>(b, pa, grid, st); break;
-      case 3: launch_pair_pers<3>(b, pa, grid, st); break;
-      case 4: launch_pair_pers<4>(b, pa, grid, st); break;
-      default: launch_pair_pers<2>(b, pa, grid, st); break;
-    }
-    LPM_HIP(hipGetLastError());
-    if (ev1) LPM_HIP(hipEventRecord(ev1, st));
-    return LPM_OK;
-  }
   switch (R) {
     case 1: launch_pair<1>(q, b, pa, grid, st); break;
     case 3: launch_pair<3>(q, b, pa, grid, st); break;
@@ -1889,7 +1806,7 @@ struct lpm_bve_ctx {
   // bumped whenever device buffers are (re)allocated
   uint64_t generation = 0;
   float near_factor = lpm::dev::kNearDefault;
-  int staging = LPM_STAGING_LDS;  // lpm_bve_set_source_staging
+  bool lds = true;
   // velocity validity: form and precision that produced u
   bool vel_valid = false;
   int vel_prec = -1, vel_form = -1;
@@ -1902,10 +1819,10 @@ struct lpm_bve_ctx {
     double dt, omega;
     int scheme, prec, R, splits;
     float near_factor;
-    int staging;
+    bool lds;
     bool operator<(const GraphKey& o) const {
-      return std::tie(dt, omega, scheme, prec, R, splits, near_factor, staging) <
-             std::tie(o.dt, o.omega, o.scheme, o.prec, o.R, o.splits, o.near_factor, o.staging);
+      return std::tie(dt, omega, scheme, prec, R, splits, near_factor, lds) <
+             std::tie(o.dt, o.omega, o.scheme, o.prec, o.R, o.splits, o.near_factor, o.lds);
     }
   };
   std::map<GraphKey, hipGraphExec_t> graphs;
@@ -2124,8 +2041,7 @@ void free_ctx_memory(lpm_bve_ctx* c) {
 Shape ctx_shape(const lpm_bve_ctx* c) {
   Shape sh = choose_shape(c->nt, std::max(c->ns, 1), c->R_req, c->splits_req);
   sh.near_factor = c->near_factor;
-  sh.lds = c->staging != LPM_STAGING_SCALAR;
-  sh.persistent = c->staging == LPM_STAGING_LDS;
+  sh.lds = c->lds;
   return sh;
 }
 
@@ -2162,7 +2078,6 @@ int ensure_eval_buffers(lpm_bve_ctx* c, const Shape& sh) {
   if ((rc = alloc((void**)&c->eb.srcq, bp.srcq))) return rc;
   if ((rc = alloc((void**)&c->eb.qtot, 256))) return rc;
   if ((rc = alloc((void**)&c->eb.part, bp.part))) return rc;
-  if ((rc = alloc((void**)&c->eb.counter, 256))) return rc;
   c->eb.part_elems = part_elems;
   ++c->generation;
   return LPM_OK;
@@ -2629,7 +2544,7 @@ int lpm_bve_advance(lpm_bve_ctx* c, double dt, double omega, int scheme, int pre
   if ((rc = ctx_prepare_step(c, omega, scheme, prec, c->stream))) return rc;
   const Shape sh = ctx_shape(c);
   if (use_graph && !c->timing && nsteps > 0) {
-    lpm_bve_ctx::GraphKey key{dt, omega, scheme, prec, sh.R, sh.splits, sh.near_factor, c->staging};
+    lpm_bve_ctx::GraphKey key{dt, omega, scheme, prec, sh.R, sh.splits, sh.near_factor, sh.lds};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       hipGraph_t g;
@@ -2911,11 +2826,9 @@ int lpm_bve_set_launch(lpm_bve_ctx* c, int targets_per_lane, int source_splits) 
   return ok();
 }
 
-int lpm_bve_set_source_staging(lpm_bve_ctx* c, int staging) {
+int lpm_bve_set_source_staging(lpm_bve_ctx* c, int lds) {
   if (!c) return fail(LPM_ERR_ARG, "lpm_bve_set_source_staging: null context");
-  if (staging != LPM_STAGING_SCALAR && staging != LPM_STAGING_LDS && staging != LPM_STAGING_LDS_GRID)
-    return fail(LPM_ERR_ARG, "lpm_bve_set_source_staging: unknown staging");
-  c->staging = staging;
+  c->lds = lds != 0;
   return ok();
 }
 

@@ -272,36 +272,6 @@ def test_graph_replay_matches_eager(omesh):
     b.close()
 
 
-def test_persistent_grid_matches_plain_grid(omesh):
-    """The FP32C velocity sums as one persistent grid (items from a counter,
-    the default) and as one workgroup per item give the same bits, at every
-    targets-per-lane count and with more split items than resident
-    workgroups, in single evaluations and in graph-replayed steps."""
-    m = omesh(6)
-    lm = LeafMesh(m)
-    vz, fz = gauss_field(m, seed=4, nvort=16, sigma=0.1, rh=(4, 0.1, 0.1))
-    fz = fz[m.fmask == 0]
-    a = make_ctx(lm, vz, fz)
-    b = make_ctx(lm, vz, fz)
-    b.set_source_staging(L.STAGING_LDS_GRID)
-    for R, splits in ((1, 0), (2, 7), (3, 0), (4, 0), (4, 32)):
-        for c in (a, b):
-            c.set_launch(R, splits)
-            c.solve(L.PREC_FP32C, L.SCHEME_LAGR)
-        sa, sb = a.get_state(), b.get_state()
-        assert np.array_equal(sa["vu"], sb["vu"]) and np.array_equal(sa["fu"], sb["fu"]), (R, splits)
-    for c in (a, b):
-        c.set_launch(0, 0)
-        c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=3)
-    sa, sb = a.get_state(), b.get_state()
-    for k in ("vx", "fx", "vu", "fu", "fzeta"):
-        assert np.array_equal(sa[k], sb[k]), k
-    with pytest.raises(L.LpmError):
-        a.set_source_staging(3)
-    a.close()
-    b.close()
-
-
 @pytest.mark.parametrize("R,splits", [(1, 1), (2, 3), (3, 0), (3, 5), (4, 0), (4, 7)])
 def test_launch_shapes_agree(omesh, R, splits):
     m = omesh(4)
