@@ -78,9 +78,17 @@ def make_ic_zero() -> VorticityInitialCondition:
     return VorticityInitialCondition()
 
 
-def pairs_per_step(level: int) -> float:
-    nl = 20 * 4 ** level
-    nv = 2 + 10 * 4 ** level
+def tree_sizes(level: int, seed: int = 0) -> tuple[int, int]:
+    """Vertices and leaves of a level-`level` tree: 2 + 10 4^L and 20 4^L
+    (icosahedral), 2 + 6 4^L and 6 4^L (cubed sphere)."""
+    if seed == SEED_CUBED_SPHERE:
+        return 2 + 6 * 4 ** level, 6 * 4 ** level
+    return 2 + 10 * 4 ** level, 20 * 4 ** level
+
+
+def pairs_per_step(level: int, seed: int = 0) -> float:
+    """Pair interactions of one RK4 step (four evaluations, self pairs excluded)."""
+    nv, nl = tree_sizes(level, seed)
     return 4.0 * ((nv + nl) * nl - nl)
 
 
@@ -120,11 +128,7 @@ def load_manifest(directory: str) -> list[Request]:
 
 
 def request_bytes(spec: MemberSpec) -> int:
-    # vertices and leaves of the level-L tree: 2 + 10 4^L and 20 4^L
-    # (icosahedral), 2 + 6 4^L and 6 4^L (cubed sphere)
-    if spec.seed == SEED_CUBED_SPHERE:
-        return state_bytes(2 + 6 * 4 ** spec.level, 6 * 4 ** spec.level)
-    return state_bytes(2 + 10 * 4 ** spec.level, 20 * 4 ** spec.level)
+    return state_bytes(*tree_sizes(spec.level, spec.seed))
 
 
 class _Lane:
@@ -320,7 +324,7 @@ class Server:
                     steps += 1
                     for r in list(lane.members):
                         r.steps_done += 1
-                        inc = pairs_per_step(r.spec.level)
+                        inc = pairs_per_step(r.spec.level, r.spec.seed)
                         r.pairs += inc
                         pairs += inc
                         if r.steps_done >= r.steps:
@@ -328,7 +332,7 @@ class Server:
                             done.append(r)
             for r in list(self.large):
                 if r.ctx.done():
-                    inc = pairs_per_step(r.spec.level) * (r.steps - r.steps_done)
+                    inc = pairs_per_step(r.spec.level, r.spec.seed) * (r.steps - r.steps_done)
                     r.steps_done = r.steps
                     r.pairs += inc
                     pairs += inc

@@ -135,6 +135,18 @@ def test_request_generator():
     assert min(r.spec.nvort for r in reqs) == 4 and max(r.spec.nvort for r in reqs) == 16
 
 
+def test_request_sizes_both_seeds():
+    """The serving loop's vertex/leaf counts and pairs per step agree with the
+    trees the library builds, for both seeds."""
+    from lpmkokkos_amd import serve as S
+    for seed, ms in ((L.SEED_ICOS_TRI, None), (L.SEED_CUBED_SPHERE, L.MeshSeed(L.CubedSphereSeed))):
+        for level in range(4):
+            m = L.TreeMesh(level, ms)
+            nl = int(np.sum(m.fmask == 0))
+            assert S.tree_sizes(level, seed) == (m.nverts, nl), (seed, level)
+            assert S.pairs_per_step(level, seed) == 4.0 * ((m.nverts + nl) * nl - nl)
+
+
 def test_poisson_y54_matches_oracle(oracle_mod):
     """Host Y_5^4 right-hand side and RH54 velocity (src/LpmSpherePoisson.hpp:60-80,
     src/LpmRossbyWaves.hpp:18-53) against the oracle's restatement: velocity bit
