@@ -459,8 +459,9 @@ int lpm_ens_set_blocks_per_cu(lpm_bve_ensemble* ens, int blocks_per_cu);
 /* No counterpart in the reference (the batching form's autotuner).
  * Time one evaluation of every member at its current state per candidate
  * persistent grid -- 1 up to the occupancy limit of workgroups (256 lanes,
- * 4 waves each) per CU -- and record the fastest as the process-wide rule
- * for ensembles of this total target count N (the members' targets summed).
+ * 4 waves each) per CU -- and record the fewest workgroups per CU within
+ * 0.5 % of the fastest as the process-wide rule for ensembles of this total
+ * target count N (the members' targets summed).
  * Results do not depend on the grid.  ms_by_blocks[k] (k < nms; NULL when
  * nms is 0) receives the mean time at k + 1 workgroups per CU, -1 where not
  * tried.  The states are left alone; each member's next step recomputes its
