@@ -70,6 +70,59 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+class ClockSampler:
+    """GFX clock of every XCD and socket power, read with amdsmi every
+    `interval` seconds in a background thread while the timed steps run.  The
+    published peaks assume 2,400 MHz; boxes run this load at lower clocks.
+    Any failure (no amdsmi, no access) leaves the figures out."""
+
+    def __init__(self, interval: float = 0.25):
+        import threading
+        self.interval = interval
+        self.samples = []  # (mean XCD clock, min, max, socket power)
+        self.error = None
+        self._stop = threading.Event()
+        self._thread = threading.Thread(target=self._run, daemon=True)
+
+    def _run(self):
+        try:
+            import amdsmi
+            amdsmi.amdsmi_init()
+            h = amdsmi.amdsmi_get_processor_handles()[0]
+            while not self._stop.is_set():
+                m = amdsmi.amdsmi_get_gpu_metrics_info(h)
+                clks = [c for c in m.get("current_gfxclks", []) if isinstance(c, (int, float)) and c > 0]
+                pw = m.get("current_socket_power")
+                if clks:
+                    self.samples.append((sum(clks) / len(clks), min(clks), max(clks),
+                                         pw if isinstance(pw, (int, float)) else None))
+                self._stop.wait(self.interval)
+            amdsmi.amdsmi_shut_down()
+        except Exception as e:  # reported, never fatal
+            self.error = repr(e)
+
+    def start(self):
+        self._thread.start()
+        return self
+
+    def stop(self):
+        self._stop.set()
+        self._thread.join(timeout=2.0)
+        if not self.samples:
+            return {"error": self.error or "no samples"}
+        n = len(self.samples)
+        pws = [x[3] for x in self.samples if x[3] is not None]
+        return {
+            "gfx_mhz_mean": sum(x[0] for x in self.samples) / n,
+            "gfx_mhz_min_xcd": min(x[1] for x in self.samples),
+            "gfx_mhz_max_xcd": max(x[2] for x in self.samples),
+            "socket_power_w_mean": sum(pws) / len(pws) if pws else None,
+            "samples": n,
+            "source": f"amdsmi current_gfxclks (8 XCDs) and current_socket_power every {self.interval} s "
+                      "during the timed FP32C steps",
+        }
+
+
 def drop_in_binding(ctx, farea, nl: int, pairs_step: int, resident_s: float, calls: int = 2):
     """The reference-side binding of advance_timestep on host arrays: copy the
     state and entry velocity in (lpm_bve_set_state + lpm_bve_set_velocity, in
@@ -325,13 +378,17 @@ def main() -> int:
     ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=args.warmup)
     ctx.synchronize()
     log(f"[bench] warmup done ({args.warmup} steps)")
+    sampler = ClockSampler().start()
     t1 = time.perf_counter()
     ctx.advance(DT, OMEGA, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=args.steps)
     ctx.synchronize()
     t_fp32c = time.perf_counter() - t1
+    clock = sampler.stop()
     ms_step = 1e3 * t_fp32c / args.steps
     rate = pairs_step * args.steps / t_fp32c
-    log(f"[bench] FP32C: {ms_step:.1f} ms/step, {rate:.4e} pairs/s")
+    clk = (f"{clock['gfx_mhz_mean']:.0f} MHz mean over the XCDs, {clock['gfx_mhz_min_xcd']}-"
+           f"{clock['gfx_mhz_max_xcd']}" if "gfx_mhz_mean" in clock else clock.get("error"))
+    log(f"[bench] FP32C: {ms_step:.1f} ms/step, {rate:.4e} pairs/s; GFX clock {clk}")
 
     # pair-kernel launch time with HIP events (one step, eager launches)
     ctx.enable_timing(True)
@@ -448,6 +505,7 @@ def main() -> int:
             "evaluations_per_step": 4,
         },
         "rk4_steps_per_s": 1e3 / ms_step,
+        "gpu_clock_during_fp32c": clock,
         "fp64_pairs_per_s": rate64,
         "fp64_reference_scheme_pairs_per_s": rate64_ref,
         "drop_in_binding_pcie_inclusive": binding,
@@ -464,6 +522,9 @@ def main() -> int:
             "peak": FP32_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
+            # the same against the peak scaled to the clock the box ran the timed steps at
+            "frac_at_measured_clock": (achieved / (FP32_VECTOR_PEAK_TFLOPS * clock["gfx_mhz_mean"] / 2400.0)
+                                       if "gfx_mhz_mean" in clock else None),
             "traffic": TRAFFIC_BYTES_PER_LAUNCH if default_launch else None,
             "traffic_source": "profiles/k_pair_f32c_lds_level8_counters.md" if default_launch else None,
             "kernel": "k_pair_f32c",
