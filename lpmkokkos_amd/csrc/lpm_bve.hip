@@ -84,12 +84,11 @@ struct Src64 {
 // Dot-form records.  Source j is target smap[j]; its area is area[smap[j]].
 // Padding records sit at the origin with q = 0 (denominator 1).  ns_dev
 // (may be null) holds the source count on the device, at most ns.
-__global__ __launch_bounds__(kBlock) void k_prep_f64_dot(Src64* __restrict__ src, Pos3 pos,
-                                                         const double* __restrict__ zeta,
-                                                         const double* __restrict__ area,
-                                                         const int* __restrict__ smap, int ns, int ns_pad,
-                                                         const int* __restrict__ ns_dev) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void prep_f64_dot_block(int blk, Src64* __restrict__ src, Pos3 pos,
+                                                   const double* __restrict__ zeta, const double* __restrict__ area,
+                                                   const int* __restrict__ smap, int ns, int ns_pad,
+                                                   const int* __restrict__ ns_dev) {
+  const int j = blk * kBlock + threadIdx.x;
   if (j >= ns_pad) return;
   if (ns_dev) ns = min(ns, *ns_dev);
   Src64 r;
@@ -104,6 +103,14 @@ __global__ __launch_bounds__(kBlock) void k_prep_f64_dot(Src64* __restrict__ src
     r.b = make_double4(0.0, 0.0, 0.0, 0.0);
   }
   src[j] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_prep_f64_dot(Src64* __restrict__ src, Pos3 pos,
+                                                         const double* __restrict__ zeta,
+                                                         const double* __restrict__ area,
+                                                         const int* __restrict__ smap, int ns, int ns_pad,
+                                                         const int* __restrict__ ns_dev) {
+  prep_f64_dot_block(blockIdx.x, src, pos, zeta, area, smap, ns, ns_pad, ns_dev);
 }
 
 // FP32C source record: a = {dx, dy, dz, q y.n}, b = {q y.e1, q y.e2, q, |d|^2}
@@ -159,13 +166,12 @@ __device__ __forceinline__ void tile_frame(const double (&c)[4], double (&n)[3],
 // tile's FP64 sum of q, its radius max |d| (rounded up), its frame (at
 // ctr[ntiles + 3t + {0,1,2}] = n, e1, e2), and the records.  Padding records
 // sit at the origin with q = 0.
-__global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, double4* __restrict__ ctr,
-                                                     float* __restrict__ trad, double* __restrict__ srcq, Pos3 pos,
-                                                     const double* __restrict__ zeta,
-                                                     const double* __restrict__ area,
-                                                     const int* __restrict__ smap, int ns) {
+__device__ __forceinline__ void prep_f32c_block(int t, int ntiles, Src32* __restrict__ src,
+                                                double4* __restrict__ ctr, float* __restrict__ trad,
+                                                double* __restrict__ srcq, Pos3 pos, const double* __restrict__ zeta,
+                                                const double* __restrict__ area, const int* __restrict__ smap,
+                                                int ns) {
   __shared__ double red[5][kTile / 64];
-  const int t = blockIdx.x;
   const int j = t * kTile + threadIdx.x;
   const bool valid = j < ns;
   double p0 = 0.0, p1 = 0.0, p2 = 0.0, q = 0.0;
@@ -222,7 +228,7 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, do
   __syncthreads();
   if (threadIdx.x == 0) {
     ctr[t] = make_double4(c[0], c[1], c[2], c[3]);
-    double4* fr = ctr + gridDim.x + 3 * (long)t;
+    double4* fr = ctr + ntiles + 3 * (long)t;
     fr[0] = make_double4(n[0], n[1], n[2], 0.0);
     fr[1] = make_double4(e1[0], e1[1], e1[2], 0.0);
     fr[2] = make_double4(e2[0], e2[1], e2[2], 0.0);
@@ -231,16 +237,24 @@ __global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, do
   }
 }
 
+__global__ __launch_bounds__(kTile) void k_prep_f32c(Src32* __restrict__ src, double4* __restrict__ ctr,
+                                                     float* __restrict__ trad, double* __restrict__ srcq, Pos3 pos,
+                                                     const double* __restrict__ zeta,
+                                                     const double* __restrict__ area,
+                                                     const int* __restrict__ smap, int ns) {
+  prep_f32c_block(blockIdx.x, gridDim.x, src, ctr, trad, srcq, pos, zeta, area, smap, ns);
+}
+
 // One workgroup per tile: centre (mean of its sources) and radius max |d|
 // (rounded up) in ctr[t], frame (at ctr[ntiles + 3t + {0,1,2}] = n, e1, e2)
 // and FP64 records; padding records sit at the origin (offset -c) with
 // q = 0, where 1 - x.y = 1 for every target x.
-__global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ src, double4* __restrict__ ctr,
-                                                          Pos3 pos, const double* __restrict__ zeta,
-                                                          const double* __restrict__ area,
-                                                          const int* __restrict__ smap, int ns) {
+__device__ __forceinline__ void prep_f64_tiles_block(int t, int ntiles, Src64* __restrict__ src,
+                                                     double4* __restrict__ ctr, Pos3 pos,
+                                                     const double* __restrict__ zeta,
+                                                     const double* __restrict__ area,
+                                                     const int* __restrict__ smap, int ns) {
   __shared__ double red[4][kTile / 64];
-  const int t = blockIdx.x;
   const int j = t * kTile + threadIdx.x;
   const bool valid = j < ns;
   double p0 = 0.0, p1 = 0.0, p2 = 0.0, q = 0.0;
@@ -286,11 +300,18 @@ __global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ sr
   if (threadIdx.x == 0) {
     const double rr = fmax(fmax(red[3][0], red[3][1]), fmax(red[3][2], red[3][3]));
     ctr[t] = make_double4(c[0], c[1], c[2], rr * (1.0 + 1e-6));
-    double4* fr = ctr + gridDim.x + 3 * (long)t;
+    double4* fr = ctr + ntiles + 3 * (long)t;
     fr[0] = make_double4(n[0], n[1], n[2], 0.0);
     fr[1] = make_double4(e1[0], e1[1], e1[2], 0.0);
     fr[2] = make_double4(e2[0], e2[1], e2[2], 0.0);
   }
+}
+
+__global__ __launch_bounds__(kTile) void k_prep_f64_tiles(Src64* __restrict__ src, double4* __restrict__ ctr,
+                                                          Pos3 pos, const double* __restrict__ zeta,
+                                                          const double* __restrict__ area,
+                                                          const int* __restrict__ smap, int ns) {
+  prep_f64_tiles_block(blockIdx.x, gridDim.x, src, ctr, pos, zeta, area, smap, ns);
 }
 
 // Sum of per-tile q sums in tile order (one workgroup; deterministic).
@@ -1097,9 +1118,7 @@ struct FinArgs {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.nt) return;
+__device__ __forceinline__ void finalize_at(const FinArgs& a, int i) {
   double S[kNC] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < a.splits; ++k)
     for (int c = 0; c < kNC; ++c) S[c] += a.part[((long)k * kNC + c) * a.nt_pad + i];
@@ -1124,6 +1143,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(FinArgs a) {
   }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < a.nt) finalize_at<MODE>(a, i);
+}
+
 // ---------------------------------------------------------------------------
 // RK4 stage arithmetic (src/LpmBVERK4_Impl.hpp:11-121)
 // ---------------------------------------------------------------------------
@@ -1146,9 +1171,7 @@ __device__ __forceinline__ void normalize3(double& a, double& b, double& c) {
   c *= s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_stage(StageArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.nt) return;
+__device__ __forceinline__ void stage_at(const StageArgs& a, int i) {
   const int s = a.stage - 1;
   const long nt = a.nt;
   // k_s = dt * u; dzeta_s = -2 Omega w dt (BVEVorticityTendency, src/LpmBVEKernels.hpp:317-319)
@@ -1193,6 +1216,70 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a) {
   }
   for (int c = 0; c < 3; ++c) X[c][i] = nx[c];
   a.zeta[i] = nz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_stage(StageArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < a.nt) stage_at(a, i);
+}
+
+// ---------------------------------------------------------------------------
+// ensemble-wide stage, prep and finalize: one launch for all members
+// ---------------------------------------------------------------------------
+
+// Per-member descriptors of one evaluation (the member's own buffers): the
+// source records' inputs and outputs, and the finalize arguments.
+struct EnsEval {
+  Src32* src32;
+  Src64* src64;
+  double4* ctr;
+  float* trad;
+  double* srcq;
+  Pos3 srcpos;
+  const double* zeta;
+  const double* area;
+  const int* smap;
+  int ns, ntiles;
+  FinArgs fin;
+};
+
+enum PrepKind { PREP_F32C = 0, PREP_F64_TILES = 1, PREP_F64_DOT = 2 };
+
+// One workgroup per (member, source tile) of the list `blk`: the member's
+// records exactly as its own prep kernel writes them.
+template <int KIND>
+__global__ __launch_bounds__(kTile) void k_prep_ens(const EnsEval* __restrict__ ev, const int2* __restrict__ blk) {
+  const int2 b = blk[blockIdx.x];
+  const EnsEval& m = ev[b.x];
+  if constexpr (KIND == PREP_F32C)
+    prep_f32c_block(b.y, m.ntiles, m.src32, m.ctr, m.trad, m.srcq, m.srcpos, m.zeta, m.area, m.smap, m.ns);
+  else if constexpr (KIND == PREP_F64_TILES)
+    prep_f64_tiles_block(b.y, m.ntiles, m.src64, m.ctr, m.srcpos, m.zeta, m.area, m.smap, m.ns);
+  else
+    prep_f64_dot_block(b.y, m.src64, m.srcpos, m.zeta, m.area, m.smap, m.ns, m.ntiles * kTile, nullptr);
+}
+
+// One workgroup per (member, target block) of the list `blk`.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_finalize_ens(const EnsEval* __restrict__ ev, const int2* __restrict__ blk) {
+  const int2 b = blk[blockIdx.x];
+  const FinArgs a = ev[b.x].fin;
+  const int i = b.y * kBlock + threadIdx.x;
+  if (i < a.nt) finalize_at<MODE>(a, i);
+}
+
+// RK4 stage `stage` of every member: one workgroup per (member, target
+// block) of the list `blk`; st[m] holds the member's arrays.
+__global__ __launch_bounds__(kBlock) void k_stage_ens(const StageArgs* __restrict__ st, const int2* __restrict__ blk,
+                                                      int stage, int lagr, double dt, double omega) {
+  const int2 b = blk[blockIdx.x];
+  StageArgs a = st[b.x];
+  a.stage = stage;
+  a.lagr = lagr;
+  a.dt = dt;
+  a.omega = omega;
+  const int i = b.y * kBlock + threadIdx.x;
+  if (i < a.nt) stage_at(a, i);
 }
 
 // absv = zeta + 2 Omega z
@@ -1557,8 +1644,7 @@ int enqueue_pair(const EvalRequest& q, const EvalBuffers& b, hipStream_t st, hip
   return LPM_OK;
 }
 
-// Sum split partials, cross product, stream function.
-int enqueue_finalize(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
+FinArgs make_fin_args(const EvalRequest& q, const EvalBuffers& b) {
   FinArgs fa;
   fa.part = b.part;
   fa.splits = q.shape.splits;
@@ -1571,6 +1657,12 @@ int enqueue_finalize(const EvalRequest& q, const EvalBuffers& b, hipStream_t st)
   fa.qtot = b.qtot;
   fa.srcq = b.srcq;
   fa.self_slot = q.self_slot;
+  return fa;
+}
+
+// Sum split partials, cross product, stream function.
+int enqueue_finalize(const EvalRequest& q, const EvalBuffers& b, hipStream_t st) {
+  const FinArgs fa = make_fin_args(q, b);
   const dim3 fg(cdiv(q.nt, kBlock));
   if (q.prec == LPM_PREC_FP32C)
     hipLaunchKernelGGL((k_finalize<FIN_F32C>), fg, dim3(kBlock), 0, st, fa);
@@ -2123,14 +2215,16 @@ int ctx_eval(lpm_bve_ctx* c, const double* px, const double* pzeta, int prec, in
   return enqueue_eval(q, c->eb, st, e0, e1);
 }
 
-int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme, hipStream_t st) {
+// The context's arrays for the stage kernel (stage, scheme, dt and Omega
+// left at defaults).
+StageArgs ctx_stage_args(const lpm_bve_ctx* c) {
   StageArgs a;
   a.nt = c->nt;
   a.nv = (int)c->nv;
-  a.stage = stage;
-  a.lagr = scheme == LPM_SCHEME_LAGR;
-  a.dt = dt;
-  a.omega = omega;
+  a.stage = 1;
+  a.lagr = 0;
+  a.dt = 0.0;
+  a.omega = 0.0;
   a.u = c->u;
   a.v = c->v;
   a.w = c->w;
@@ -2147,6 +2241,15 @@ int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme, hi
   a.wy = c->wy;
   a.wz = c->wz;
   a.wzeta = c->wzeta;
+  return a;
+}
+
+int ctx_stage(lpm_bve_ctx* c, int stage, double dt, double omega, int scheme, hipStream_t st) {
+  StageArgs a = ctx_stage_args(c);
+  a.stage = stage;
+  a.lagr = scheme == LPM_SCHEME_LAGR;
+  a.dt = dt;
+  a.omega = omega;
   hipLaunchKernelGGL(k_stage, dim3(cdiv(c->nt, kBlock)), dim3(kBlock), 0, st, a);
   LPM_HIP(hipGetLastError());
   return LPM_OK;
@@ -2904,7 +3007,16 @@ struct lpm_bve_ensemble {
   // runs on the GPU must not free.
   lpm::dev::EnsMember* d_mem_w = nullptr;  // descriptors: stage arguments as targets
   lpm::dev::EnsMember* d_mem_x = nullptr;  // descriptors: state as targets
+  lpm::dev::EnsEval* d_eval_w = nullptr;   // prep / finalize descriptors, the same two kinds
+  lpm::dev::EnsEval* d_eval_x = nullptr;
+  lpm::dev::StageArgs* d_stage = nullptr;  // each member's arrays for the stage kernel
   size_t cap_mem = 0;
+  // (member, target block) and (member, source tile) lists of the
+  // ensemble-wide stage, finalize and prep launches
+  int2* d_tblk = nullptr;
+  int2* d_sblk = nullptr;
+  size_t cap_tblk = 0, cap_sblk = 0;
+  int n_tblk = 0, n_sblk = 0;
   std::vector<lpm::dev::EnsItem*> item_bufs;  // by launch index
   std::vector<size_t> item_caps;
   int* d_counters = nullptr;  // four per persistent launch in a step
@@ -2932,7 +3044,12 @@ void ens_free_buffers(lpm_bve_ensemble* e) {
   for (void* p : e->plan_allocs) (void)hipFree(p);
   e->plan_allocs.clear();
   e->d_mem_w = e->d_mem_x = nullptr;
+  e->d_eval_w = e->d_eval_x = nullptr;
+  e->d_stage = nullptr;
   e->cap_mem = 0;
+  e->d_tblk = e->d_sblk = nullptr;
+  e->cap_tblk = e->cap_sblk = 0;
+  e->n_tblk = e->n_sblk = 0;
   e->item_bufs.clear();
   e->item_caps.clear();
   e->d_counters = nullptr;
@@ -2965,9 +3082,13 @@ int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
   e->plan_per_cu = ens_blocks_per_cu(e);
   const size_t M = e->members.size();
   std::vector<EnsMember> hw(M), hx(M);
+  std::vector<EnsEval> vw(M), vx(M);
+  std::vector<StageArgs> hs(M);
+  std::vector<int2> tblk, sblk;
   std::map<int, std::vector<std::pair<long, EnsItem>>> byR;
   for (size_t m = 0; m < M; ++m) {
     lpm_bve_ctx* c = e->members[m];
+    hs[m] = ctx_stage_args(c);
     for (int which = 0; which < 2; ++which) {
       EvalRequest q;
       int rc = ctx_request(c, which ? c->x : c->wx, which ? c->zeta : c->wzeta, prec, form, false, q);
@@ -2978,7 +3099,22 @@ int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
       d.ctr = c->eb.ctr;
       d.trad = c->eb.trad;
       d.src64 = c->eb.src64;
+      EnsEval& v = which ? vx[m] : vw[m];
+      v.src32 = c->eb.src32;
+      v.src64 = c->eb.src64;
+      v.ctr = c->eb.ctr;
+      v.trad = c->eb.trad;
+      v.srcq = c->eb.srcq;
+      v.srcpos = q.srcpos;
+      v.zeta = q.zeta;
+      v.area = q.area;
+      v.smap = q.smap;
+      v.ns = q.ns;
+      v.ntiles = q.shape.ntiles;
+      v.fin = make_fin_args(q, c->eb);
       if (which) {
+        for (int k = 0; k < cdiv(c->nt, kBlock); ++k) tblk.push_back(make_int2((int)m, k));
+        for (int t = 0; t < q.shape.ntiles; ++t) sblk.push_back(make_int2((int)m, t));
         const Shape& sh = q.shape;
         const int nbx = cdiv(c->nt, (long)kBlock * sh.R);
         for (int by = 0; by < sh.splits; ++by) {
@@ -3002,13 +3138,33 @@ int ens_build_plan(lpm_bve_ensemble* e, int prec, int form) {
     const size_t cap = std::max<size_t>(M, 2 * e->cap_mem);
     if ((rc = alloc((void**)&e->d_mem_w, sizeof(EnsMember) * cap))) return rc;
     if ((rc = alloc((void**)&e->d_mem_x, sizeof(EnsMember) * cap))) return rc;
+    if ((rc = alloc((void**)&e->d_eval_w, sizeof(EnsEval) * cap))) return rc;
+    if ((rc = alloc((void**)&e->d_eval_x, sizeof(EnsEval) * cap))) return rc;
+    if ((rc = alloc((void**)&e->d_stage, sizeof(StageArgs) * cap))) return rc;
     e->cap_mem = cap;
   }
+  if (tblk.size() > e->cap_tblk) {
+    const size_t cap = std::max(tblk.size(), 2 * e->cap_tblk);
+    if ((rc = alloc((void**)&e->d_tblk, sizeof(int2) * cap))) return rc;
+    e->cap_tblk = cap;
+  }
+  if (sblk.size() > e->cap_sblk) {
+    const size_t cap = std::max(sblk.size(), 2 * e->cap_sblk);
+    if ((rc = alloc((void**)&e->d_sblk, sizeof(int2) * cap))) return rc;
+    e->cap_sblk = cap;
+  }
+  e->n_tblk = (int)tblk.size();
+  e->n_sblk = (int)sblk.size();
   if (!e->d_counters && (rc = alloc((void**)&e->d_counters, sizeof(int) * 4 * 4))) return rc;
   // stream-ordered uploads: the previous step (which reads these buffers) is
   // ahead of them on the ensemble's stream
   LPM_HIP(hipMemcpyAsync(e->d_mem_w, hw.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice, e->stream));
   LPM_HIP(hipMemcpyAsync(e->d_mem_x, hx.data(), sizeof(EnsMember) * M, hipMemcpyHostToDevice, e->stream));
+  LPM_HIP(hipMemcpyAsync(e->d_eval_w, vw.data(), sizeof(EnsEval) * M, hipMemcpyHostToDevice, e->stream));
+  LPM_HIP(hipMemcpyAsync(e->d_eval_x, vx.data(), sizeof(EnsEval) * M, hipMemcpyHostToDevice, e->stream));
+  LPM_HIP(hipMemcpyAsync(e->d_stage, hs.data(), sizeof(StageArgs) * M, hipMemcpyHostToDevice, e->stream));
+  LPM_HIP(hipMemcpyAsync(e->d_tblk, tblk.data(), sizeof(int2) * tblk.size(), hipMemcpyHostToDevice, e->stream));
+  LPM_HIP(hipMemcpyAsync(e->d_sblk, sblk.data(), sizeof(int2) * sblk.size(), hipMemcpyHostToDevice, e->stream));
   size_t k = 0;
   // one host list per launch, all alive until the uploads below have run (a
   // copy from pageable memory may still be queued behind the running step)
@@ -3109,18 +3265,21 @@ int ens_grid(lpm_bve_ensemble* e, int R, int prec, int form, int nitems) {
   return std::max(1, std::min(nitems, per_cu * prop.multiProcessorCount));
 }
 
-// One evaluation of every member: prep per member, one persistent pair
-// launch per distinct R, finalize per member.  which: 0 = stage arguments,
-// 1 = state.
+// One evaluation of every member: one prep launch over all members' source
+// tiles, one persistent pair launch per distinct R over all members' items,
+// one finalize launch over all members' targets.  which: 0 = stage
+// arguments, 1 = state.
 int ens_eval(lpm_bve_ensemble* e, int which, int prec, int form, int pass) {
   int rc;
-  std::vector<EvalRequest> qs(e->members.size());
-  for (size_t m = 0; m < e->members.size(); ++m) {
-    lpm_bve_ctx* c = e->members[m];
-    if ((rc = ctx_request(c, which ? c->x : c->wx, which ? c->zeta : c->wzeta, prec, form, false, qs[m]))) return rc;
-    qs[m].shape.lds = true;  // the persistent kernels stage sources through LDS
-    if ((rc = enqueue_prep(qs[m], c->eb, e->stream))) return rc;
-  }
+  const EnsEval* ev = which ? e->d_eval_x : e->d_eval_w;
+  const dim3 sg(e->n_sblk), tg(e->n_tblk);
+  if (prec == LPM_PREC_FP32C)
+    hipLaunchKernelGGL(k_prep_ens<PREP_F32C>, sg, dim3(kTile), 0, e->stream, ev, e->d_sblk);
+  else if (form == FORM_DOT)
+    hipLaunchKernelGGL(k_prep_ens<PREP_F64_DOT>, sg, dim3(kTile), 0, e->stream, ev, e->d_sblk);
+  else
+    hipLaunchKernelGGL(k_prep_ens<PREP_F64_TILES>, sg, dim3(kTile), 0, e->stream, ev, e->d_sblk);
+  LPM_HIP(hipGetLastError());
   const EnsMember* mem = which ? e->d_mem_x : e->d_mem_w;
   for (size_t k = 0; k < e->plan_R.size(); ++k) {
     int* counter = e->d_counters + 4 * k + pass;
@@ -3132,8 +3291,13 @@ int ens_eval(lpm_bve_ensemble* e, int which, int prec, int form, int pass) {
     else rc = ens_launch_R<2>(e, (int)k, mem, counter, prec, form, grid);
     if (rc) return rc;
   }
-  for (size_t m = 0; m < e->members.size(); ++m)
-    if ((rc = enqueue_finalize(qs[m], e->members[m]->eb, e->stream))) return rc;
+  if (prec == LPM_PREC_FP32C)
+    hipLaunchKernelGGL(k_finalize_ens<FIN_F32C>, tg, dim3(kBlock), 0, e->stream, ev, e->d_tblk);
+  else if (form == FORM_DOT)
+    hipLaunchKernelGGL(k_finalize_ens<FIN_DOT>, tg, dim3(kBlock), 0, e->stream, ev, e->d_tblk);
+  else
+    hipLaunchKernelGGL(k_finalize_ens<FIN_CHORD>, tg, dim3(kBlock), 0, e->stream, ev, e->d_tblk);
+  LPM_HIP(hipGetLastError());
   return LPM_OK;
 }
 
@@ -3141,8 +3305,9 @@ int ens_enqueue_step(lpm_bve_ensemble* e, double dt, double omega, int scheme, i
   const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
   int rc;
   for (int stage = 1; stage <= 4; ++stage) {
-    for (auto* c : e->members)
-      if ((rc = ctx_stage(c, stage, dt, omega, scheme, e->stream))) return rc;
+    hipLaunchKernelGGL(k_stage_ens, dim3(e->n_tblk), dim3(kBlock), 0, e->stream, e->d_stage, e->d_tblk, stage,
+                       scheme == LPM_SCHEME_LAGR ? 1 : 0, dt, omega);
+    LPM_HIP(hipGetLastError());
     if ((rc = ens_eval(e, stage == 4 ? 1 : 0, prec, form, stage - 1))) return rc;
   }
   return LPM_OK;
