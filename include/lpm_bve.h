@@ -439,11 +439,13 @@ int lpm_bve_autotune(lpm_bve_ctx* ctx, int prec, int scheme, int reps, int* best
 /* Ensemble: independent problems (contexts, possibly of different     */
 /* mesh levels) advanced together.  Each of the four evaluations of a  */
 /* step runs as ONE persistent pair kernel over the flattened list of  */
-/* (member, target block, source split) items, longest first; each     */
-/* member keeps the decomposition it has alone and writes its own      */
-/* partial slots, so its results are bit-identical alone and in the    */
-/* ensemble.  Members are borrowed, must stay alive while they belong  */
-/* to the ensemble, and must not be stepped on their own concurrently. */
+/* (member, target block, source split) items, longest first (one per  */
+/* targets-per-lane count), with one stage, one source-record and one  */
+/* finalize launch over all members; each member keeps the             */
+/* decomposition it has alone and writes its own partial slots, so its */
+/* results are bit-identical alone and in the ensemble.  Members are   */
+/* borrowed, must stay alive while they belong to the ensemble, and    */
+/* must not be stepped on their own concurrently.                      */
 /* ------------------------------------------------------------------ */
 typedef struct lpm_bve_ensemble lpm_bve_ensemble;
 
