@@ -58,7 +58,7 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 # HBM bytes per level-8 FP32C pair launch at the default launch shape: FETCH_SIZE + WRITE_SIZE
 # from two rocprofv3 --pmc runs, each scaled by the same counter's reading for a device copy of
 # known size
-TRAFFIC_BYTES_PER_LAUNCH = 1.3997e9 + 1.1347e9
+TRAFFIC_BYTES_PER_LAUNCH = 4.1810e9 + 4.3496e9
 # FP32C/FP64 speed-up per step allowed by the pair loops' instruction issue
 # (22.2 vs 45.0 SIMD cycles per 64 pairs, measured issue costs)
 FP32C_VS_FP64_ISSUE_BOUND = 45.0 / 22.2

@@ -1503,13 +1503,17 @@ Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
   int splits = splits_req;
   if (splits <= 0) {
     // at least 2048 workgroups (8 per CU on 256 CUs) with >= 4 tiles per
-    // split; on large problems, short workgroups: about 32768 of them with
-    // >= 64 tiles each, at most 32 splits.  Workgroup durations vary, so the
-    // last ones to finish leave CUs idle for about one workgroup's duration:
+    // split; on large problems, short workgroups: about 131072 of them with
+    // >= 64 tiles each, at most 72 splits and about 4.6 GB of split partials
+    // (32 bytes per target and split).  Workgroup durations vary, so the last
+    // ones to finish leave CUs idle for about one workgroup's duration, and
+    // another stream's kernels (a serving lane's) wait for workgroup slots:
     // with 1.31M sources and 1.97M targets the FP32C launch took 457 ms with
-    // 2 splits (workgroups of ~116 ms) and 421-424 ms with 16 to 33.
+    // 2 splits (workgroups of ~116 ms), 400 ms with 18 and 396-397 ms with
+    // 36 or 72, and level-5 requests beside it finished 33 % sooner with 72.
     const int fill = std::min(cdiv(2048, wg_x), std::max(1, sh.ntiles / 4));
-    const int tail = std::min({cdiv(32768, wg_x), sh.ntiles / 64, 32});
+    const int budget = (int)std::max(1.0, 4.6e9 / (32.0 * (double)nt));
+    const int tail = std::min({cdiv(131072, wg_x), sh.ntiles / 64, 72, budget});
     splits = std::max({1, fill, tail});
   }
   splits = std::min(splits, sh.ntiles);
