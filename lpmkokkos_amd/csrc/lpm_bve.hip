@@ -3615,7 +3615,7 @@ int lpm_bve_autotune(lpm_bve_ctx* c, int prec, int scheme, int reps, int* best_R
   std::vector<std::pair<int, int>> cands;
   for (int R : {1, 2, 3, 4}) {
     const Shape base = choose_shape(c->nt, std::max(c->ns, 1), R, 0);
-    std::vector<int> sp = {base.splits, 1, 2, 4, 8, 16, 32};
+    std::vector<int> sp = {base.splits, 1, 2, 4, 8, 16, 32, 64};
     std::sort(sp.begin(), sp.end());
     sp.erase(std::unique(sp.begin(), sp.end()), sp.end());
     for (int s : sp)
