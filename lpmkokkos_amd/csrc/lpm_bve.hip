@@ -1483,7 +1483,9 @@ std::map<int, ShapeRule> g_rules;
 // workgroups (of kBlock lanes, 4 waves) per CU, from lpm_ens_autotune.
 std::map<int64_t, int> g_ens_rules;
 
-Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
+// short_items: contexts split the sources finely (short workgroups); the
+// functor entry points keep coarser splits, which bound their scratch.
+Shape choose_shape(int nt, int ns, int R_req, int splits_req, bool short_items = true) {
   if (R_req == 0) {
     std::lock_guard<std::mutex> g(g_rule_mu);
     auto it = g_rules.find(nt);
@@ -1503,17 +1505,21 @@ Shape choose_shape(int nt, int ns, int R_req, int splits_req) {
   int splits = splits_req;
   if (splits <= 0) {
     // at least 2048 workgroups (8 per CU on 256 CUs) with >= 4 tiles per
-    // split; on large problems, short workgroups: about 131072 of them with
-    // >= 64 tiles each, at most 72 splits and about 4.6 GB of split partials
-    // (32 bytes per target and split).  Workgroup durations vary, so the last
-    // ones to finish leave CUs idle for about one workgroup's duration, and
-    // another stream's kernels (a serving lane's) wait for workgroup slots:
-    // with 1.31M sources and 1.97M targets the FP32C launch took 457 ms with
-    // 2 splits (workgroups of ~116 ms), 400 ms with 18 and 396-397 ms with
-    // 36 or 72, and level-5 requests beside it finished 33 % sooner with 72.
+    // split; then short workgroups: about 131072 of them with >= 4 tiles
+    // each, at most 72 splits and about 4.6 GB of split partials (32 bytes
+    // per target and split).  Workgroup durations vary, so the last ones to
+    // finish leave CUs idle for about one workgroup's duration, and another
+    // stream's kernels (a serving lane's) wait for workgroup slots: with 1.31M
+    // sources and 1.97M targets the FP32C launch took 457 ms with 2 splits
+    // (workgroups of ~116 ms), 400 ms with 18 and 396-397 ms with 36 or 72,
+    // and level-5 requests beside it finished 33 % sooner with 72; a level-7
+    // FP32C step took 118 ms with 20 splits and 111 ms with 72, a level-6 one
+    // 9.2 ms with 9 and 7.7 ms with 72.  Functor launches keep the earlier
+    // rule: about 32768 workgroups of >= 64 tiles, at most 32 splits.
     const int fill = std::min(cdiv(2048, wg_x), std::max(1, sh.ntiles / 4));
     const int budget = (int)std::max(1.0, 4.6e9 / (32.0 * (double)nt));
-    const int tail = std::min({cdiv(131072, wg_x), sh.ntiles / 64, 72, budget});
+    const int tail = short_items ? std::min({cdiv(131072, wg_x), sh.ntiles / 4, 72, budget})
+                                 : std::min({cdiv(32768, wg_x), sh.ntiles / 64, 32});
     splits = std::max({1, fill, tail});
   }
   splits = std::min(splits, sh.ntiles);
@@ -1719,7 +1725,7 @@ struct FunctorPlan {
 
 FunctorPlan functor_plan(int64_t nt, int64_t nf, bool need_u) {
   FunctorPlan p;
-  p.shape = choose_shape((int)nt, (int)std::max<int64_t>(nf, 1), 2, 0);
+  p.shape = choose_shape((int)nt, (int)std::max<int64_t>(nf, 1), 2, 0, false);
   const BufferPlan bp = plan_buffers((int)nt, p.shape);
   size_t off = 0;
   auto take = [&](size_t n) {

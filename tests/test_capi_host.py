@@ -99,13 +99,13 @@ def test_config3_members_build():
     assert E.state_bytes(655362, 1310720) > 29 * 8 * (655362 + 1310720)
 
 
-@pytest.mark.parametrize("level,splits", [(6, 9), (7, 20), (8, 69), (9, 18)])
+@pytest.mark.parametrize("level,splits", [(4, 5), (5, 20), (6, 64), (7, 72), (8, 69), (9, 18)])
 def test_state_bytes_count_the_default_split_partials(level, splits):
     """The memory a context plans (and serving admission charges) holds the
     default launch's split partials: 4 FP64 sums per target and source split,
-    with 69 splits at level 8 and 20 at level 7 (short workgroups on large
-    problems), 9 at level 6, and 18 at level 9, where the 4.6 GB bound on the
-    partials sets the count."""
+    with short workgroups: 64 and 72 splits at levels 6 and 7, 69 at level 8, 18 at
+    level 9, where the 4.6 GB bound on the partials sets the count, and at
+    least 4 tiles per split below (5 and 20 at levels 4 and 5)."""
     from lpmkokkos_amd import ensemble as E
     nv, na = 2 + 10 * 4 ** level, 20 * 4 ** level
     nt = nv + na
