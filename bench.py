@@ -213,7 +213,9 @@ def ensemble_run(nmembers: int = 64, nsteps: int = 10) -> dict:
     """Batching configuration: 64 members, levels uniform on 3..7 (seed 3), RH
     wave (K x U(0.95, 1.05)) + 8 Gaussian vortices, 10 FP32C steps, one
     persistent pair kernel per evaluation, step graph replayed, launch shapes
-    and the persistent grid (workgroups per CU) from the autotuners.  Every member is then advanced alone by the same 10
+    from the autotuner (per level alone, then re-tuned on the whole ensemble
+    for the levels that carry 90 % of the pairs) and the persistent grid
+    (workgroups per CU) from the grid autotuner.  Every member is then advanced alone by the same 10
     steps (one at a time, as a caller without batching would run them: the
     sequential rate) and compared with its ensemble result bit for bit."""
     import lpmkokkos_amd as L
@@ -230,6 +232,11 @@ def ensemble_run(nmembers: int = 64, nsteps: int = 10) -> dict:
     ens = E.Ensemble(0)
     for c in members:
         ens.add(c)
+    # shapes of the sizes carrying 90 % of the pairs re-tuned on whole-ensemble evaluations
+    # (the rule for N, which the members alone below take too)
+    ntuned = ens.autotune_shapes(L.PREC_FP32C, L.SCHEME_LAGR, reps=1, work_fraction=0.9)
+    ens_shapes = {s.level: E.shape_rule(c.sizes()[3]) for s, c in zip(specs, members)}
+    log(f"[bench] ensemble: shapes of {ntuned} size(s) re-tuned on the ensemble: {ens_shapes}")
     # waves per CU of the persistent kernel: the grid autotuner's rule for this total N
     grid = ens.autotune(L.PREC_FP32C, L.SCHEME_LAGR, reps=1)
     log(f"[bench] ensemble: persistent grid autotuned: {grid[0]} workgroups per CU "
@@ -274,6 +281,7 @@ def ensemble_run(nmembers: int = 64, nsteps: int = 10) -> dict:
         "members_bitwise_equal_alone": f"{same}/{len(specs)}",
         "all_members_bitwise_equal_alone": same == len(specs),
         "autotuned_shapes": {int(k): list(v) for k, v in tuned.items()},
+        "ensemble_tuned_shapes": {int(k): list(v) for k, v in ens_shapes.items() if v},
         "autotuned_workgroups_per_cu": grid[0],
         "eval_ms_by_workgroups_per_cu": grid[2],
     }

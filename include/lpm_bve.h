@@ -472,6 +472,18 @@ int lpm_ens_autotune(lpm_bve_ensemble* ens, int prec, int scheme, int reps, int*
                      double* best_ms, double* ms_by_blocks, int nms);
 /* Forget every ensemble grid rule. */
 int lpm_ens_rules_clear(void);
+/* No counterpart in the reference (the batching form's autotuner).
+ * Launch shapes tuned for the ensemble: for the member sizes N that carry the
+ * most pair work, largest first, until `work_fraction` of the ensemble's pair
+ * work is covered, time one evaluation of every member at its state per
+ * candidate shape of that N (targets per lane x source splits) and record
+ * the fastest as the process-wide rule for N (lpm_shape_rule_get).  A
+ * context of that N run alone takes the same rule, so results stay
+ * bit-identical alone and in the ensemble.  Members must have automatic
+ * launch shapes.  *ntuned (may be NULL) receives the number of sizes tuned;
+ * each member's next step recomputes its entry velocity. */
+int lpm_ens_autotune_shapes(lpm_bve_ensemble* ens, int prec, int scheme, int reps, double work_fraction,
+                            int* ntuned);
 /* `nsteps` RK4 steps of every member (same dt, omega, scheme, precision). */
 int lpm_ens_advance(lpm_bve_ensemble* ens, double dt, double omega, int scheme, int prec, int nsteps,
                     int use_graph);

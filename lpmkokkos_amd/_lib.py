@@ -115,6 +115,7 @@ SIGNATURES = {
     "lpm_ens_autotune": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                          C.c_void_p, C.c_int],
     "lpm_ens_rules_clear": [],
+    "lpm_ens_autotune_shapes": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)],
     "lpm_ens_advance": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
     "lpm_ens_synchronize": [C.c_void_p],
     "lpm_ens_query": [C.c_void_p, C.POINTER(C.c_int)],

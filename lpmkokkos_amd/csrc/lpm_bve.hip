@@ -3483,6 +3483,117 @@ int lpm_ens_autotune(lpm_bve_ensemble* e, int prec, int scheme, int reps, int* b
   return ok();
 }
 
+// Launch shapes tuned for the ensemble: for the member sizes N that carry the
+// most pair work (until `work_fraction` of it is covered), try launch shapes
+// for that N on one evaluation of the whole ensemble and keep the fastest as
+// the rule for N.  Members of one N share a rule, and a context run alone
+// takes the same rule, so results stay bit-identical alone and in the
+// ensemble.
+int lpm_ens_autotune_shapes(lpm_bve_ensemble* e, int prec, int scheme, int reps, double work_fraction,
+                            int* ntuned) {
+  if (!e) return fail(LPM_ERR_ARG, "lpm_ens_autotune_shapes: null ensemble");
+  if (scheme != LPM_SCHEME_REF && scheme != LPM_SCHEME_LAGR)
+    return fail(LPM_ERR_ARG, "lpm_ens_autotune_shapes: bad scheme");
+  if (prec != LPM_PREC_FP64 && prec != LPM_PREC_FP32C)
+    return fail(LPM_ERR_ARG, "lpm_ens_autotune_shapes: bad precision");
+  if (prec == LPM_PREC_FP32C && scheme != LPM_SCHEME_LAGR)
+    return fail(LPM_ERR_ARG, "lpm_ens_autotune_shapes: FP32C pair sums require the renormalised (LAGR) scheme");
+  if (!(work_fraction > 0.0)) return fail(LPM_ERR_ARG, "lpm_ens_autotune_shapes: work_fraction must be > 0");
+  if (e->members.empty()) return fail(LPM_ERR_STATE, "lpm_ens_autotune_shapes: no members");
+  if (reps < 1) reps = 1;
+  LPM_HIP(hipSetDevice(e->device));
+  const int form = scheme == LPM_SCHEME_LAGR ? FORM_CHORD : FORM_DOT;
+  for (auto* c : e->members) {
+    if (c->timing) return fail(LPM_ERR_STATE, "lpm_ens_autotune_shapes: member has pair timing enabled");
+    if (c->R_req || c->splits_req)
+      return fail(LPM_ERR_STATE, "lpm_ens_autotune_shapes: member has an explicit launch shape");
+    if (c->stream == e->stream || mark_reached(c->done_flag, c->done_seq)) continue;
+    LPM_HIP(hipEventRecord(e->ev, c->stream));
+    LPM_HIP(hipStreamWaitEvent(e->stream, e->ev, 0));
+  }
+  // pair work per member size N, largest first
+  std::map<int, double> work;
+  std::map<int, int> nsrc;
+  double total = 0.0;
+  for (auto* c : e->members) {
+    const double w = (double)c->nt * c->ns;
+    work[c->nt] += w;
+    nsrc[c->nt] = c->ns;
+    total += w;
+  }
+  std::vector<std::pair<double, int>> order;
+  for (auto& kv : work) order.push_back({kv.second, kv.first});
+  std::sort(order.rbegin(), order.rend());
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) {
+    if (t0) (void)hipEventDestroy(t0);
+    return fail(LPM_ERR_HIP, "lpm_ens_autotune_shapes: event creation failed");
+  }
+  auto time_eval = [&](double* ms_out) -> int {
+    int rc;
+    for (auto* c : e->members)
+      if ((rc = ensure_eval_buffers(c, ctx_shape(c)))) return rc;
+    if ((rc = ens_build_plan(e, prec, form))) return rc;
+    if ((rc = ens_eval(e, 1, prec, form, 0))) return rc;  // warm
+    LPM_HIP(hipEventRecord(t0, e->stream));
+    for (int r = 0; r < reps; ++r)
+      if ((rc = ens_eval(e, 1, prec, form, 0))) return rc;
+    LPM_HIP(hipEventRecord(t1, e->stream));
+    LPM_HIP(hipEventSynchronize(t1));
+    float ms = 0.0f;
+    LPM_HIP(hipEventElapsedTime(&ms, t0, t1));
+    *ms_out = ms / reps;
+    return LPM_OK;
+  };
+  int rc = LPM_OK, tuned = 0;
+  double covered = 0.0;
+  for (auto& ow : order) {
+    if (covered >= work_fraction * total) break;
+    covered += ow.first;
+    const int N = ow.second;
+    const int ns = nsrc[N];
+    const Shape base = choose_shape(N, std::max(ns, 1), 0, 0);
+    std::vector<std::pair<int, int>> cands;
+    for (int R : {base.R, 2, 4}) {
+      for (int sp : {base.splits, 8, 16, 32, 64}) {
+        const Shape sh = choose_shape(N, std::max(ns, 1), R, sp);
+        const std::pair<int, int> c2(sh.R, sh.splits);
+        if (std::find(cands.begin(), cands.end(), c2) == cands.end()) cands.push_back(c2);
+      }
+    }
+    double best = 1e300;
+    std::pair<int, int> bestc = cands.front();
+    for (auto& cd : cands) {
+      {
+        std::lock_guard<std::mutex> g(g_rule_mu);
+        g_rules[N] = ShapeRule{cd.first, cd.second};
+      }
+      double ms = 0.0;
+      if ((rc = time_eval(&ms))) break;
+      if (ms < best) {
+        best = ms;
+        bestc = cd;
+      }
+    }
+    {
+      std::lock_guard<std::mutex> g(g_rule_mu);
+      g_rules[N] = ShapeRule{bestc.first, bestc.second};
+    }
+    if (rc) break;
+    ++tuned;
+  }
+  (void)hipStreamSynchronize(e->stream);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  ens_drop_plan(e);
+  // the trial evaluations overwrote each member's velocity; its next step
+  // recomputes the entry velocity
+  for (auto* c : e->members) c->vel_valid = false;
+  if (rc) return rc;
+  if (ntuned) *ntuned = tuned;
+  return ok();
+}
+
 int lpm_ens_rules_clear(void) {
   std::lock_guard<std::mutex> g(g_rule_mu);
   g_ens_rules.clear();

@@ -74,6 +74,16 @@ class Ensemble:
              max_blocks)
         return best.value, ms.value, {k + 1: float(t) for k, t in enumerate(times) if t >= 0}
 
+    def autotune_shapes(self, prec: int = PREC_FP32C, scheme: int = SCHEME_LAGR, reps: int = 1,
+                        work_fraction: float = 0.9) -> int:
+        """Launch shapes tuned on whole-ensemble evaluations for the member
+        sizes N carrying the most pair work (until work_fraction of it is
+        covered); each becomes the process-wide rule for its N, which contexts
+        run alone take too.  Returns the number of sizes tuned."""
+        n = C.c_int()
+        call("lpm_ens_autotune_shapes", self.h, prec, scheme, reps, float(work_fraction), C.byref(n))
+        return n.value
+
     def advance(self, dt: float, omega: float, scheme: int = SCHEME_LAGR, prec: int = PREC_FP32C, nsteps: int = 1,
                 use_graph: bool = True):
         call("lpm_ens_advance", self.h, dt, omega, scheme, prec, nsteps, int(use_graph))

@@ -178,6 +178,34 @@ def test_grid_autotune_changes_no_bits():
             c.close()
 
 
+def test_ensemble_shape_autotune_keeps_members_alone():
+    """Shapes tuned on whole-ensemble evaluations become the rules for the
+    tuned sizes N, which members run alone take too: every member still
+    equals its run alone bit for bit."""
+    from lpmkokkos_amd import _lib
+    sp = specs()
+    mixed = [E.make_member(s) for s in sp]
+    ens = E.Ensemble(0)
+    for c in mixed:
+        ens.add(c)
+    n = ens.autotune_shapes(L.PREC_FP32C, L.SCHEME_LAGR, reps=1, work_fraction=1.0)
+    sizes = sorted({c.sizes()[3] for c in mixed})
+    assert n == len(sizes) and all(E.shape_rule(N) is not None for N in sizes)
+    ens.advance(0.005, TWO_PI, nsteps=3)
+    alone = [E.make_member(s) for s in sp]
+    for c in alone:
+        c.advance(0.005, TWO_PI, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=3)
+    for i, (a, b) in enumerate(zip(alone, mixed)):
+        assert same(a, b), i
+    mixed[0].set_launch(1, 2)
+    with pytest.raises(RuntimeError):
+        ens.autotune_shapes()  # a member with an explicit launch shape
+    _lib.call("lpm_shape_rules_clear")
+    ens.close()
+    for c in alone + mixed:
+        c.close()
+
+
 def test_grid_autotune_errors():
     ens = E.Ensemble(0)
     with pytest.raises(RuntimeError):
