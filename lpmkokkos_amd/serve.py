@@ -150,7 +150,10 @@ class Server:
     change between steps -- so a small request is held to the steps of
     requests of its own size, not to a level-7 step.  A request at
     `large_level` or above runs in a large lane (one of `large_lanes` streams,
-    all its steps queued at once).  The GPU interleaves the lanes' workgroups.
+    all its steps queued at once).  One large lane is the default: large
+    requests then run one after another, so the first of two finishes in about
+    half the time it takes when both share the device.  The GPU interleaves the
+    lanes' workgroups.
 
     Every lane's stream is created once, at construction: the runtime shares
     its hardware queues (GPU_MAX_HW_QUEUES, 4 unless set before the first HIP
@@ -161,7 +164,7 @@ class Server:
 
     def __init__(self, device: int = 0, budget_bytes: float = 200e9, max_active: int = 256, dt: float = 0.005,
                  omega: float = 2 * math.pi, prec: int = PREC_FP32C, keep_results: bool = False,
-                 log=None, large_level: int = 8, large_lanes: int = 2, batch_levels=(5, 6, 7)):
+                 log=None, large_level: int = 8, large_lanes: int = 1, batch_levels=(5, 6, 7)):
         self.device = device
         self.budget = budget_bytes
         self.max_active = max_active
@@ -451,6 +454,8 @@ def main(argv=None) -> int:
     ap.add_argument("--checkpoint-dir", default=None)
     ap.add_argument("--resume", default=None, metavar="DIR",
                     help="serve the requests a drained server left in DIR (its manifest.json)")
+    ap.add_argument("--large-lanes", type=int, default=1,
+                    help="lanes for level-8 requests (one runs them one after another, two side by side)")
     args = ap.parse_args(argv)
     if args.stop_after_steps is not None and not args.checkpoint_dir:
         ap.error("--stop-after-steps needs --checkpoint-dir")
@@ -467,7 +472,7 @@ def main(argv=None) -> int:
         with open(os.path.join(args.resume, "manifest.json")) as f:
             saved = json.load(f)
         srv = Server(budget_bytes=args.budget_gb * 1e9, log=log, dt=saved["dt"], omega=saved["omega"],
-                     prec=saved["prec"])
+                     prec=saved["prec"], large_lanes=args.large_lanes)
         res = srv.run(reqs, stop_after_steps=args.stop_after_steps, checkpoint_dir=args.checkpoint_dir)
         srv.close()
         res.update({"resumed_from": args.resume, "budget_gb": args.budget_gb})
@@ -478,7 +483,7 @@ def main(argv=None) -> int:
     mean_service = sum(probs[l] * svc[l] for l in probs)
     rate = args.load / mean_service
     reqs = make_requests(args.requests, rate, seed=args.seed, level_probs=probs, steps=args.steps)
-    srv = Server(budget_bytes=args.budget_gb * 1e9, log=log)
+    srv = Server(budget_bytes=args.budget_gb * 1e9, log=log, large_lanes=args.large_lanes)
     res = srv.run(reqs, stop_after_steps=args.stop_after_steps, checkpoint_dir=args.checkpoint_dir)
     srv.close()
     res.update({"arrival_rate_per_s": rate, "service_s_by_level": svc, "load": args.load,

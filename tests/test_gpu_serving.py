@@ -17,8 +17,8 @@ def test_requests_match_alone_under_admission_pressure():
     reqs = S.make_requests(10, rate=200.0, seed=11, level_probs={2: 0.4, 3: 0.4, 4: 0.2}, steps=3)
     biggest = max(S.request_bytes(r.spec) for r in reqs)
     # room for about three requests at a time: later arrivals wait for retirements
-    # level-4 requests take the large lane, the rest share the batched lane
-    srv = S.Server(budget_bytes=3.2 * biggest, keep_results=True, large_level=4)
+    # level-4 requests take the two large lanes, the rest share the batched lane
+    srv = S.Server(budget_bytes=3.2 * biggest, keep_results=True, large_level=4, large_lanes=2)
     res = srv.run(reqs)
     srv.close()
     assert res["requests"] == 10
