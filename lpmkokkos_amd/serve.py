@@ -14,8 +14,9 @@ admission needs no host mesh, no host field evaluation and no upload.
 Each request's result is bit-identical to running it alone: a member keeps
 the decomposition it has alone and writes only its own partial slots.
 
-Requests below level 8 share the batched lane; level-8 requests run in lanes
-of their own (see Server), so short requests are not held to a level-8 step.
+Requests below level 8 are batched in lanes by size class; level-8 requests
+run in a large lane of their own (see Server), so short requests are not held
+to a level-8 step.
 
 Checkpoint and resume: run(..., stop_after_steps=k, checkpoint_dir=d) stops
 admitting after k batched steps, lets the large lanes finish, writes every
@@ -170,6 +171,8 @@ class Server:
         self.max_active = max_active
         self.dt, self.omega, self.prec = dt, omega, prec
         self.keep_results = keep_results
+        if large_lanes < 0:
+            raise ValueError(f"large_lanes must be >= 0, got {large_lanes}")
         self.large_level = large_level
         bounds = sorted({b for b in batch_levels if b < large_level - 1} | {large_level - 1})
         self.lanes = [_Lane(device, lo, hi) for lo, hi in zip([0] + [b + 1 for b in bounds[:-1]], bounds)]

@@ -199,3 +199,10 @@ def test_velocity_tangent():
     assert np.abs(d).max() < 1e-15
     d2 = L.velocity_tangent(m.vx, m.vx)
     assert np.abs(d2 - 1).max() < 1e-15
+
+
+def test_server_rejects_negative_large_lanes():
+    """Checked before any device work, so it runs without a GPU."""
+    from lpmkokkos_amd import serve as S
+    with pytest.raises(ValueError):
+        S.Server(large_lanes=-1)
