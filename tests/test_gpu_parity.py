@@ -346,6 +346,35 @@ def test_level8_lagrangian_subset():
     c.close()
 
 
+@pytest.mark.timeout(300)
+def test_level9_subset():
+    """The largest size run here: level 9 (2,621,442 vertices, 5,242,880 faces,
+    4.1e13 pairs per evaluation), where the bound on the split partials, not the
+    workgroup length, sets the default launch (18 source splits).  The same 32
+    Gaussian vortices: FP64 Lagrangian-scheme velocity against the oracle's
+    chord-form sums on every 19997th target (1e-12), FP32C against FP64 at
+    every target (1e-6)."""
+    from oracle.oracle import Mesh
+    m = Mesh(9)
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=1, nvort=32, sigma=0.1)
+    fz = fz[m.fmask == 0]
+    c = make_ctx(lm, vz, fz)
+    print(f"level 9: {lm.vx.shape[0]} vertices, {fz.size} faces", flush=True)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR)
+    a = c.get_state()
+    stride = 19997
+    ov = O.velocity_chord(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False, stride=stride)[::stride]
+    of = O.velocity_chord(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True, stride=stride)[::stride]
+    ev, ef = normwise(a["vu"][::stride], ov), normwise(a["fu"][::stride], of)
+    c.solve(L.PREC_FP32C, L.SCHEME_LAGR)
+    b = c.get_state()
+    e32 = max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"]))
+    print(f"level 9 subset: FP64 chord vertex {ev:.3e} face {ef:.3e}; FP32C vs FP64 {e32:.3e}")
+    assert ev < 1e-12 and ef < 1e-12 and e32 < 1e-6
+    c.close()
+
+
 def test_solid_body_rotation_gpu(omesh):
     """tests/LpmBVETest.cpp: zeta = 2 Omega z moves particles rigidly.  The
     renormalised scheme tracks the exact rotation at level 4; the reference
