@@ -160,3 +160,27 @@ def test_large_request_without_large_lanes_is_refused():
     with pytest.raises(ValueError):
         srv.run(reqs)
     srv.close()
+
+
+def test_mixed_seed_requests_match_alone():
+    """Cubed-sphere and icosahedral requests share the size-class lanes: each
+    is sized from its own seed (memory charge, pair count) and its result equals
+    the request run alone, bit for bit."""
+    cs = L.SEED_CUBED_SPHERE
+    specs = [E.MemberSpec(level=3, seed=cs, rh_n=3, vortex_seed=70), E.MemberSpec(level=3, vortex_seed=71),
+             E.MemberSpec(level=4, seed=cs, nvort=5, vortex_seed=72), E.MemberSpec(level=2, vortex_seed=73),
+             E.MemberSpec(level=4, vortex_seed=74)]
+    reqs = [S.Request(rid=k, arrival=0.01 * k, spec=s, steps=4) for k, s in enumerate(specs)]
+    srv = S.Server(keep_results=True, large_level=5, batch_levels=(3,))
+    res = srv.run(reqs)
+    srv.close()
+    assert res["requests"] == 5 and res["batch_lanes"] == [[0, 3], [4, 4]] and res["large_lane_requests"] == 0
+    assert res["pairs"] == sum(4 * S.pairs_per_step(s.level, s.seed) for s in specs)
+    for r in reqs:
+        c = E.make_member(r.spec)
+        c.advance(0.005, 2 * math.pi, L.SCHEME_LAGR, L.PREC_FP32C, nsteps=r.steps)
+        s = c.get_state()
+        assert s["vx"].shape[0] == S.tree_sizes(r.spec.level, r.spec.seed)[0], r.rid
+        for k in ("vx", "vzeta", "vu", "fx", "fzeta", "fu"):
+            assert np.array_equal(s[k], r.result[k]), (r.rid, k)
+        c.close()
