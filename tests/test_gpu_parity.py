@@ -375,6 +375,37 @@ def test_level9_subset():
     c.close()
 
 
+def test_cubed_sphere_level8_subset():
+    """The cubed-sphere seed at level 8 (393,218 vertices, 393,216 quadrilateral
+    leaves), 32 Gaussian vortices: FP64 Lagrangian-scheme velocity against the
+    oracle's chord-form sums on every 997th target (1e-12), FP32C against FP64
+    at every target (1e-6), the reference scheme's literal 1 - x.y against the
+    oracle on the same subset (1e-11)."""
+    m = O.Mesh(8, kind="cubed")
+    lm = LeafMesh(m)
+    vz, fz = gauss_field(m, seed=1, nvort=32, sigma=0.1)
+    fz = fz[m.fmask == 0]
+    c = make_ctx(lm, vz, fz)
+    c.solve(L.PREC_FP64, L.SCHEME_LAGR)
+    a = c.get_state()
+    stride = 997
+    ov = O.velocity_chord(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False, stride=stride)[::stride]
+    of = O.velocity_chord(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True, stride=stride)[::stride]
+    ev, ef = normwise(a["vu"][::stride], ov), normwise(a["fu"][::stride], of)
+    c.solve(L.PREC_FP32C, L.SCHEME_LAGR)
+    b = c.get_state()
+    e32 = max(normwise(b["vu"], a["vu"]), normwise(b["fu"], a["fu"]))
+    c.solve(L.PREC_FP64, L.SCHEME_REF)
+    r = c.get_state()
+    rv = O.velocity(lm.vx, lm.fx, fz, lm.farea, lm.fmask, False, stride=stride)[::stride]
+    rf = O.velocity(lm.fx, lm.fx, fz, lm.farea, lm.fmask, True, stride=stride)[::stride]
+    erv, erf = normwise(r["vu"][::stride], rv), normwise(r["fu"][::stride], rf)
+    print(f"cubed level 8 subset: FP64 chord vertex {ev:.3e} face {ef:.3e}; FP32C vs FP64 {e32:.3e}; "
+          f"FP64 dot vertex {erv:.3e} face {erf:.3e}")
+    assert ev < 1e-12 and ef < 1e-12 and e32 < 1e-6 and erv < 1e-11 and erf < 1e-11
+    c.close()
+
+
 def test_solid_body_rotation_gpu(omesh):
     """tests/LpmBVETest.cpp: zeta = 2 Omega z moves particles rigidly.  The
     renormalised scheme tracks the exact rotation at level 4; the reference
