@@ -1043,13 +1043,17 @@ struct EnsItem {
   int member, bx, by;
 };
 
+// At the ensemble kernels' issue priority, so that a stream's small kernels
+// are not held behind another stream's ensemble waves.
 __global__ void k_reset_counter(int* counter) {
+  __builtin_amdgcn_s_setprio(3);
   if (threadIdx.x == 0) *counter = 0;
 }
 
 // Completion marker: lane 0 writes `seq` to host-visible (pinned) memory after
 // everything queued before it on the stream.
 __global__ void k_mark(volatile unsigned long long* flag, unsigned long long seq) {
+  __builtin_amdgcn_s_setprio(3);  // as k_reset_counter
   if (threadIdx.x == 0) {
     __threadfence_system();
     *flag = seq;
