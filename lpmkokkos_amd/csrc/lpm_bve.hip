@@ -1043,17 +1043,13 @@ struct EnsItem {
   int member, bx, by;
 };
 
-// At the ensemble kernels' issue priority, so that a stream's small kernels
-// are not held behind another stream's ensemble waves.
 __global__ void k_reset_counter(int* counter) {
-  __builtin_amdgcn_s_setprio(3);
   if (threadIdx.x == 0) *counter = 0;
 }
 
 // Completion marker: lane 0 writes `seq` to host-visible (pinned) memory after
 // everything queued before it on the stream.
 __global__ void k_mark(volatile unsigned long long* flag, unsigned long long seq) {
-  __builtin_amdgcn_s_setprio(3);  // as k_reset_counter
   if (threadIdx.x == 0) {
     __threadfence_system();
     *flag = seq;
@@ -1080,14 +1076,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_f32c_ens(const EnsMember* __res
   __shared__ Src32 sh[2][kTile];
   __shared__ double xs[R >= 3 ? 3 * R * kBlock : 1];
   __shared__ int slot;
-  // Ensemble waves take issue priority over the waves of a context's own
-  // launches on the same SIMD (the ensemble kernels all run at the same
-  // priority, so among themselves nothing changes).  A serving loop steps its
-  // batched ensembles beside a large request's launches, which fill every CU:
-  // an ensemble of three level-5 members then steps in 5.1 ms instead of 10.9
-  // (2.1 ms alone), while the large request loses the same device time per
-  // batched step as before.  Results do not depend on it.
-  __builtin_amdgcn_s_setprio(3);
   for (;;) {
     const int it = next_item(counter, &slot);
     if (it >= nitems) break;
@@ -1103,7 +1091,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_f64_ens(const EnsMember* __rest
                                                          int* counter) {
   __shared__ Src64 sh[2][kTile];
   __shared__ int slot;
-  __builtin_amdgcn_s_setprio(3);  // as k_pair_f32c_ens
   for (;;) {
     const int it = next_item(counter, &slot);
     if (it >= nitems) break;
@@ -1239,7 +1226,6 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a) {
 // ---------------------------------------------------------------------------
 // ensemble-wide stage, prep and finalize: one launch for all members
 // ---------------------------------------------------------------------------
-// (Their waves run at the ensemble pair kernels' issue priority.)
 
 // Per-member descriptors of one evaluation (the member's own buffers): the
 // source records' inputs and outputs, and the finalize arguments.
@@ -1263,7 +1249,6 @@ enum PrepKind { PREP_F32C = 0, PREP_F64_TILES = 1, PREP_F64_DOT = 2 };
 // records exactly as its own prep kernel writes them.
 template <int KIND>
 __global__ __launch_bounds__(kTile) void k_prep_ens(const EnsEval* __restrict__ ev, const int2* __restrict__ blk) {
-  __builtin_amdgcn_s_setprio(3);
   const int2 b = blk[blockIdx.x];
   const EnsEval& m = ev[b.x];
   if constexpr (KIND == PREP_F32C)
@@ -1277,7 +1262,6 @@ __global__ __launch_bounds__(kTile) void k_prep_ens(const EnsEval* __restrict__ 
 // One workgroup per (member, target block) of the list `blk`.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_finalize_ens(const EnsEval* __restrict__ ev, const int2* __restrict__ blk) {
-  __builtin_amdgcn_s_setprio(3);
   const int2 b = blk[blockIdx.x];
   const FinArgs a = ev[b.x].fin;
   const int i = b.y * kBlock + threadIdx.x;
@@ -1288,7 +1272,6 @@ __global__ __launch_bounds__(kBlock) void k_finalize_ens(const EnsEval* __restri
 // block) of the list `blk`; st[m] holds the member's arrays.
 __global__ __launch_bounds__(kBlock) void k_stage_ens(const StageArgs* __restrict__ st, const int2* __restrict__ blk,
                                                       int stage, int lagr, double dt, double omega) {
-  __builtin_amdgcn_s_setprio(3);
   const int2 b = blk[blockIdx.x];
   StageArgs a = st[b.x];
   a.stage = stage;
