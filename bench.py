@@ -489,6 +489,11 @@ def main() -> int:
 
     serving = None
     if args.serve_requests > 0:
+        # the serving loop runs with its own default launch shapes and grids, as
+        # `python -m lpmkokkos_amd.serve` does: forget the process-wide rules the
+        # batching configuration tuned for its sizes
+        L._lib.call("lpm_shape_rules_clear")
+        L._lib.call("lpm_ens_rules_clear")
         serving = serving_form(args.serve_requests, args.serve_max_level)
         log(f"[bench] serving sample: p50 {serving['latency_p50_s']:.2f} s, p99 {serving['latency_p99_s']:.2f} s, "
             f"{serving['pairs_per_s']:.4e} pairs/s")
